@@ -1,0 +1,86 @@
+"""Builds every native library of charming_amd in-tree with hipcc for gfx950.
+
+Outputs go to charming_amd/lib/ (git-ignored, but shipped with the tree to
+the GPU host).  Run as `python -m charming_amd.build`.
+"""
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+LIBDIR = os.path.join(PKG, "lib")
+OBJDIR = os.path.join(PKG, "_build")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+if not os.path.exists(HIPCC):
+    HIPCC = shutil.which("hipcc") or HIPCC
+
+# gfx950 only; no FMA contraction (bit-exact FP64 against the host oracle);
+# the packed-FP32 instructions are disabled.
+DEVICE_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off", "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument", "-Wno-unused-result",
+    "-I", INCLUDE, "-I", CSRC,
+]
+
+# (library name, sources, extra link libraries from this build)
+LIBS = [
+    ("libchm_jacobi2d.so", ["jacobi2d/plain.hip", "jacobi2d/persistent_stub.cpp"],
+     []),
+]
+
+
+def _obj(src):
+    return os.path.join(OBJDIR, src.replace("/", "__") + ".o")
+
+
+def _compile(src, verbose):
+    path = os.path.join(CSRC, src)
+    cmd = [HIPCC] + DEVICE_FLAGS + ["-c", path, "-o", _obj(src)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    out = "\n".join(l for l in (p.stdout + p.stderr).splitlines()
+                    if "packed-fp32-ops" not in l and l.strip())
+    if p.returncode != 0:
+        raise RuntimeError(f"hipcc failed on {src}:\n{out}")
+    return src
+
+
+def _link(name, srcs, deps, verbose):
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o",
+           os.path.join(LIBDIR, name)] + [_obj(s) for s in srcs]
+    cmd += ["-L", LIBDIR] + [f"-l{d}" for d in deps]
+    cmd += ["-Wl,-rpath,$ORIGIN", "-lpthread"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError(f"link failed for {name}:\n{p.stdout}{p.stderr}")
+
+
+def build(verbose=False, jobs=None):
+    os.makedirs(LIBDIR, exist_ok=True)
+    os.makedirs(OBJDIR, exist_ok=True)
+    srcs = sorted({s for _, ss, _ in LIBS for s in ss})
+    missing = [s for s in srcs if not os.path.exists(os.path.join(CSRC, s))]
+    if missing:
+        raise FileNotFoundError(f"missing sources: {missing}")
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        for f in cf.as_completed([ex.submit(_compile, s, verbose) for s in srcs]):
+            f.result()
+    for name, ss, deps in LIBS:  # in order: the runtime first
+        _link(name, ss, deps, verbose)
+    return LIBDIR
+
+
+if __name__ == "__main__":
+    build(verbose="-v" in sys.argv)
+    print("built into", LIBDIR)

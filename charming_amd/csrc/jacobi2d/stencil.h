@@ -1,0 +1,287 @@
+// FP64 5-point Jacobi sweep for gfx950, shared by every form of the solver
+// (plain per-launch, hipGraph, message-driven persistent, batched, served).
+//
+// Arithmetic follows examples/jacobi2d/jacobi2d.cu:327-335 exactly:
+//   u'[i][j] = (u[i][j] + u[i][j-1] + u[i][j+1] + u[i-1][j] + u[i+1][j]) * 0.2
+// evaluated left to right, compiled with -ffp-contract=off.
+//
+// Array layout (one array = one chare block or one whole grid):
+//   (h + 2) rows of `pitch` doubles; row i holds the halo column 0, interior
+//   columns 1..w and the halo column w+1 at element offsets 15 .. 16 + w, so
+//   interior column 1 starts on a 128-byte boundary and every wave load of
+//   two doubles per lane (dwordx4) is aligned and coalesced.  Rows 0 and h+1
+//   are the halo rows.  pitch = roundup(w + 18, 16) leaves a spare element
+//   after the right halo so a pair load that starts on column w+1 stays in
+//   the row.
+//
+// Work decomposition: one wavefront sweeps a panel of NC chunks (a chunk is
+// 64 lanes x 2 doubles = 128 columns) down a band of rows.  Each lane keeps
+// the rows above/at/below in registers (a rolling window), so every input
+// element is loaded once per panel; the west/east neighbours come from the
+// adjacent lane (cross-lane moves), from the adjacent chunk (readlane) or,
+// at the panel's two ends, from one extra element load by lane 0 / lane 63.
+// Loads for the next U rows are issued before the current U rows are
+// computed, so each wave keeps U * NC KiB of reads in flight.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace chm {
+namespace jac {
+
+constexpr int kColOff = 15;  // element offset of halo column 0 within a row
+
+__host__ __device__ inline int64_t pitch_for(int w) {
+  return ((int64_t)w + 18 + 15) / 16 * 16;
+}
+__host__ __device__ inline size_t array_elems(int w, int h) {
+  return (size_t)(h + 2) * (size_t)pitch_for(w);
+}
+
+// ----------------------------------------------------------------------------
+// exact fixed-point residual: sum floor((a-b)^2 * 2^56) in 128 bits
+// ----------------------------------------------------------------------------
+struct u128acc {
+  uint64_t lo, hi;
+};
+__device__ inline void acc_add(u128acc& a, uint64_t v) {
+  uint64_t lo = a.lo + v;
+  a.hi += (lo < a.lo) ? 1u : 0u;
+  a.lo = lo;
+}
+__device__ inline void acc_add(u128acc& a, u128acc b) {
+  uint64_t lo = a.lo + b.lo;
+  a.hi += b.hi + ((lo < a.lo) ? 1u : 0u);
+  a.lo = lo;
+}
+__device__ inline uint64_t res_term(double a, double b) {
+  double d = a - b;
+  double q = d * d * 0x1.0p56;
+  if (!(q < 0x1.0p64)) return ~0ull;
+  return (uint64_t)q;
+}
+
+// Exact 128-bit add into a global accumulator {lo, hi} with two 64-bit
+// atomics: the carry out of the low word is added to the high word, so the
+// final value is the exact sum whatever the interleaving.
+__device__ inline void atomic_add_u128(uint64_t* dst, u128acc v) {
+  uint64_t old = __hip_atomic_fetch_add(dst, v.lo, __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+  uint64_t carry = (old + v.lo < old) ? 1u : 0u;
+  uint64_t hi = v.hi + carry;
+  if (hi)
+    __hip_atomic_fetch_add(dst + 1, hi, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ----------------------------------------------------------------------------
+// splitmix64 counter-based input and order-independent grid hash
+// ----------------------------------------------------------------------------
+__host__ __device__ inline uint64_t sm64_mix(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__host__ __device__ inline uint64_t cell_key(int64_t row, int64_t col) {
+  return ((uint64_t)row << 32) | (uint64_t)(uint32_t)col;
+}
+__host__ __device__ inline double cell_u01(uint64_t seed, int64_t row,
+                                           int64_t col) {
+  uint64_t z = seed + (cell_key(row, col) + 1) * 0x9E3779B97F4A7C15ull;
+  return (double)(sm64_mix(z) >> 11) * 0x1.0p-53;
+}
+__device__ inline uint64_t hash_cell(int64_t row, int64_t col, double v) {
+  return sm64_mix((uint64_t)__double_as_longlong(v) +
+                  cell_key(row, col) * 0x9E3779B97F4A7C15ull);
+}
+
+// ----------------------------------------------------------------------------
+// cross-lane helpers (wave64)
+// ----------------------------------------------------------------------------
+__device__ inline double readlane_d(double v, int lane) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __builtin_amdgcn_readlane(lo, lane);
+  hi = __builtin_amdgcn_readlane(hi, lane);
+  return __hiloint2double(hi, lo);
+}
+// value of lane-1 (lane 0 gets garbage: callers overwrite it)
+__device__ inline double from_left_lane(double v) { return __shfl_up(v, 1, 64); }
+// value of lane+1 (lane 63 gets garbage: callers overwrite it)
+__device__ inline double from_right_lane(double v) {
+  return __shfl_down(v, 1, 64);
+}
+
+// Where a sweep writes the outgoing halo edges of its result (chare form).
+// Each pointer is the first double of an edge (nullptr = not wanted).
+struct EdgeOut {
+  double* top;     // row 1, w values
+  double* bottom;  // row h, w values
+  double* left;    // column 1, h values
+  double* right;   // column w, h values
+};
+
+template <int NC>
+struct RowRegs {
+  double2 v[NC];
+  double eL, eR;  // lane 0: column c0-1; lane 63: column c0+128*NC
+};
+
+template <int NC>
+struct Panel {
+  const double* in;
+  double* out;
+  int64_t pitch;
+  int w, h;    // interior of the array
+  int c0;      // first interior column of the panel (1-based)
+  int lane;
+  bool right_edge;  // the panel's east neighbour column is loaded separately
+  int col[NC];
+  bool ld[NC], st2[NC], st1[NC];
+
+  __device__ inline void setup(const double* in_, double* out_, int64_t p,
+                               int w_, int h_, int c0_) {
+    in = in_;
+    out = out_;
+    pitch = p;
+    w = w_;
+    h = h_;
+    c0 = c0_;
+    lane = __lane_id();
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      int c = c0 + 128 * k + 2 * lane;
+      col[k] = c;
+      ld[k] = c <= w + 1;
+      st2[k] = c + 1 <= w;
+      st1[k] = c == w;
+    }
+    right_edge = (c0 + 128 * NC) <= w + 1;
+  }
+
+  __device__ inline void load(RowRegs<NC>& r, int row) const {
+    const double* rp = in + (int64_t)row * pitch + kColOff;
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      if (ld[k])
+        r.v[k] = *reinterpret_cast<const double2*>(rp + col[k]);
+      else
+        r.v[k] = make_double2(0.0, 0.0);
+    }
+    r.eL = (lane == 0) ? rp[c0 - 1] : 0.0;
+    r.eR = (lane == 63 && right_edge) ? rp[c0 + 128 * NC] : 0.0;
+  }
+
+  template <bool RES, bool PACK>
+  __device__ inline void compute(const RowRegs<NC>& up, const RowRegs<NC>& cu,
+                                 const RowRegs<NC>& dn, int row,
+                                 u128acc& acc, const EdgeOut& eo) const {
+    double* orow = out + (int64_t)row * pitch + kColOff;
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      const double a = cu.v[k].x, b = cu.v[k].y;
+      double bl = from_left_lane(b);
+      double ar = from_right_lane(a);
+      if (lane == 0) bl = (k == 0) ? cu.eL : readlane_d(cu.v[k - 1].y, 63);
+      if (lane == 63)
+        ar = (k == NC - 1) ? cu.eR : readlane_d(cu.v[(k + 1) % NC].x, 0);
+      // c + W + E + N + S, then * 0.2 (jacobi2d.cu:332-333)
+      const double na = (a + bl + b + up.v[k].x + dn.v[k].x) * 0.2;
+      const double nb = (b + a + ar + up.v[k].y + dn.v[k].y) * 0.2;
+      const int c = col[k];
+      if (st2[k]) {
+        *reinterpret_cast<double2*>(orow + c) = make_double2(na, nb);
+        if (RES) {
+          acc_add(acc, res_term(na, a));
+          acc_add(acc, res_term(nb, b));
+        }
+      } else if (st1[k]) {
+        orow[c] = na;
+        if (RES) acc_add(acc, res_term(na, a));
+      }
+      if (PACK) {
+        if (st2[k] || st1[k]) {
+          if (row == 1 && eo.top) {
+            eo.top[c - 1] = na;
+            if (st2[k]) eo.top[c] = nb;
+          }
+          if (row == h && eo.bottom) {
+            eo.bottom[c - 1] = na;
+            if (st2[k]) eo.bottom[c] = nb;
+          }
+          if (c == 1 && eo.left) eo.left[row - 1] = na;
+          if (eo.right) {
+            if (c == w) eo.right[row - 1] = na;
+            if (st2[k] && c + 1 == w) eo.right[row - 1] = nb;
+          }
+        }
+      }
+    }
+  }
+};
+
+// Sweep rows [r0, r1) (1-based, r1 <= h+1) of the panel starting at column
+// c0.  Reads rows r0-1 .. r1 of `in`, writes rows r0 .. r1-1 of `out`.
+template <int NC, int U, bool RES, bool PACK>
+__device__ inline void sweep_panel(const double* __restrict__ in,
+                                   double* __restrict__ out, int64_t pitch,
+                                   int w, int h, int c0, int r0, int r1,
+                                   u128acc& acc, const EdgeOut& eo) {
+  if (r0 >= r1) return;
+  Panel<NC> p;
+  p.setup(in, out, pitch, w, h, c0);
+  RowRegs<NC> up, cu, A[U], B[U];
+  p.load(up, r0 - 1);
+  p.load(cu, r0);
+#pragma unroll
+  for (int u = 0; u < U; u++)
+    if (r0 + 1 + u <= r1) p.load(A[u], r0 + 1 + u);
+
+  int r = r0;
+  for (;;) {
+    // prefetch the group after this one, then compute this one (in A)
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (r + U + 1 + u <= r1) p.load(B[u], r + U + 1 + u);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (r + u < r1) {
+        const RowRegs<NC>& c = (u == 0) ? cu : A[u - 1];
+        const RowRegs<NC>& n = (u == 0) ? up : ((u == 1) ? cu : A[u - 2]);
+        p.template compute<RES, PACK>(n, c, A[u], r + u, acc, eo);
+      }
+    }
+    up = (U == 1) ? cu : A[U - 2];
+    cu = A[U - 1];
+    r += U;
+    if (r >= r1) break;
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (r + U + 1 + u <= r1) p.load(A[u], r + U + 1 + u);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (r + u < r1) {
+        const RowRegs<NC>& c = (u == 0) ? cu : B[u - 1];
+        const RowRegs<NC>& n = (u == 0) ? up : ((u == 1) ? cu : B[u - 2]);
+        p.template compute<RES, PACK>(n, c, B[u], r + u, acc, eo);
+      }
+    }
+    up = (U == 1) ? cu : B[U - 2];
+    cu = B[U - 1];
+    r += U;
+    if (r >= r1) break;
+  }
+}
+
+// Wave-level reduction of a u128 accumulator to lane 0 (exact).
+__device__ inline u128acc wave_reduce_u128(u128acc a) {
+  for (int off = 32; off >= 1; off >>= 1) {
+    u128acc o;
+    o.lo = __shfl_down(a.lo, off, 64);
+    o.hi = __shfl_down(a.hi, off, 64);
+    acc_add(a, o);
+  }
+  return a;
+}
+
+}  // namespace jac
+}  // namespace chm

@@ -1,0 +1,142 @@
+"""Pins the CPU oracle against published vectors, hand-derived known answers
+and closed forms (the reference ships no fixtures or tests of its own)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+def test_splitmix64_published_vector():
+    # splitmix64 from state 0 (Vigna's splitmix64.c): first three outputs.
+    assert O.splitmix64_stream(0, 3) == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4,
+                                         0x06C45D188009454F]
+
+
+def test_cell_u01_is_counter_based_and_uniform():
+    # value at (row, col) is the first splitmix64 output from state
+    # seed + key*golden, key = (row << 32) | col, mapped by (z >> 11) * 2^-53
+    seed, row, col = 12345, 7, 9
+    key = (row << 32) | col
+    state = (seed + key * 0x9E3779B97F4A7C15) % 2**64
+    z = O.splitmix64_stream(state, 1)[0]
+    assert O.cell_u01(seed, row, col) == (z >> 11) * 2.0**-53
+    g = O.init_grid(256, 256, seed=1)
+    inner = g[1:-1, 1:-1]
+    assert inner.min() >= 0.0 and inner.max() < 1.0
+    assert abs(inner.mean() - 0.5) < 0.01
+    assert np.all(g[0, :] == 1.0) and np.all(g[:, 0] == 1.0)
+    assert np.all(g[-1, :] == 1.0) and np.all(g[:, -1] == 1.0)
+
+
+def test_constant_field_is_fixed_point():
+    g = np.ones((34, 18))
+    assert np.array_equal(O.step(g), g)  # (1+1+1+1+1)*0.2 == 1.0 exactly
+
+
+def test_reference_init_known_answers():
+    # init_kernel (jacobi2d.cu:282-293): interior 0, ring 10
+    W, H = 6, 5
+    g1, _ = O.jacobi_run(W, H, 1, mode=1)
+    inner = g1[1:-1, 1:-1]
+    exp = np.zeros((H, W))
+    exp[0, :] += 2.0; exp[-1, :] += 2.0; exp[:, 0] += 2.0; exp[:, -1] += 2.0
+    assert np.array_equal(inner, exp)  # edges 10*0.2 = 2, corners 20*0.2 = 4
+    assert inner[0, 0] == 4.0 and inner[0, 1] == 2.0 and inner[1, 1] == 0.0
+    # second sweep by hand, reference order c + W + E + N + S then * 0.2
+    g2, _ = O.jacobi_run(W, H, 2, mode=1)
+    assert g2[1, 1] == (4.0 + 10.0 + 2.0 + 10.0 + 2.0) * 0.2 == 5.6000000000000005
+    assert g2[1, 2] == (2.0 + 4.0 + 2.0 + 10.0 + 0.0) * 0.2
+    assert g2[2, 2] == (0.0 + 2.0 + 0.0 + 2.0 + 0.0) * 0.2 == 0.8
+
+
+def test_operand_order_and_multiply_by_point_two():
+    # ((((c + W) + E) + N) + S) * 0.2 with values where any other order differs
+    g = np.zeros((3, 3))
+    g[1, 1], g[1, 0], g[1, 2], g[0, 1], g[2, 1] = 1e16, 1.0, 1.0, -1e16, 0.0
+    assert O.step(g)[1, 1] == 0.0           # c+N first would give 0.4
+    g[:] = 0.0
+    g[1, 1], g[1, 0], g[1, 2] = 1.0, 1.0, 1.0   # sum 3
+    assert O.step(g)[1, 1] == 0.6000000000000001  # 3*0.2, not 3/5 == 0.6
+    g[:] = 0.0
+    g[1, 1], g[0, 1] = 1e16, 1.0      # 1e16+... + N(1.0) rounds away, S=1
+    g[2, 1] = 1.0
+    assert O.step(g)[1, 1] == 1e16 * 0.2
+
+
+@pytest.mark.parametrize("W,H,cr,cc,iters,pes", [
+    (64, 64, 4, 4, 20, 4), (37, 29, 3, 4, 25, 5), (50, 8, 1, 5, 9, 3),
+    (16, 16, 1, 1, 7, 1), (33, 65, 8, 2, 13, 16)])
+def test_decomposition_invariance(W, H, cr, cc, iters, pes):
+    g, r = O.jacobi_run(W, H, iters, seed=11)
+    g2, r2, _ = O.jacobi_chares_run(W, H, cr, cc, iters, pes, seed=11)
+    assert np.array_equal(g, g2)
+    assert r == r2
+
+
+def test_reference_init_through_chares():
+    g, r = O.jacobi_run(40, 24, 6, mode=1)
+    g2, r2, _ = O.jacobi_chares_run(40, 24, 3, 4, 6, 4, mode=1)
+    assert np.array_equal(g, g2) and r == r2
+
+
+def test_residual_closed_form():
+    a = np.floor(O.init_grid(20, 10, seed=3) * 1024) / 1024  # dyadic: +0.5 exact
+    b = a.copy()
+    b[1:-1, 1:-1] += 0.5
+    # (a-b)^2 = 0.25 = 2^-2 exactly: each cell floor(2^-2 * 2^56) = 2^54
+    assert O.residual_fixed(a, b) == 200 * 2**54
+    b = a.copy()
+    b[1:-1, 1:-1] = a[1:-1, 1:-1] + 2.0**-30 * (a[1:-1, 1:-1] * 0 + 1)
+    # differences are tiny; floor truncates every term to 0 or a small integer
+    s = O.residual_fixed(a, b)
+    assert 0 <= s <= 200
+    c = np.zeros((3, 3)); d = np.zeros((3, 3)); d[1, 1] = 10.0
+    assert O.residual_fixed(c, d) == 100 * 2**56    # max possible term
+    assert O.u128_to_double(2**64 + 5) == 2.0**64 + 5.0
+    assert O.u128_to_double(3 * 2**70) == 3 * 2.0**70
+
+
+def test_grid_hash_properties():
+    g = O.init_grid(30, 20, seed=5)
+    h = O.grid_hash(g)
+    parts = (O.grid_hash(g, 1, 11) + O.grid_hash(g, 11, 21)) % 2**64
+    assert h == parts
+    g2 = g.copy()
+    g2[3, 4] = np.nextafter(g2[3, 4], 2.0)
+    assert O.grid_hash(g2) != h
+    g3 = g.copy()
+    g3[3, 4], g3[4, 3] = g[4, 3], g[3, 4]  # swap two values: position matters
+    assert O.grid_hash(g3) != h
+
+
+def test_decompose_matches_surface_rule():
+    assert O.decompose(16, 512, 512) == (4, 4)
+    assert O.decompose(256, 16384, 16384) == (16, 16)
+    assert O.decompose(12, 400, 300) == (3, 4)   # surf 400 beats (4,3) 416.7
+    assert O.decompose(7, 100, 100) == (1, 7)
+
+
+def test_block_map():
+    assert list(O.block_map(10, 4)) == [0, 0, 0, 1, 1, 1, 2, 2, 3, 3]
+    assert list(O.block_map(2, 4, [1, 0, 1, 0])) == [0, 2]
+    assert list(O.block_map(3, 8)) == [0, 1, 2]
+
+
+def test_wire_sizes():
+    assert O.alloc_size(0, 1032) == 1072     # 512^2 halo, 4x4 chares
+    assert O.alloc_size(0, 8200) == 8240     # 16384^2 halo, 16x16 chares
+    assert O.alloc_size(0, 8) == 48
+    assert O.alloc_size(0, 1 << 20) == 1048608
+    assert O.alloc_size(0, 0) == 32
+    assert O.alloc_size(5, 0) == 16
+
+
+def test_solve_tol_stops_at_threshold():
+    g, n, s_last, s0 = O.jacobi_solve_tol(4, 4, 500, 1e-6, seed=2)
+    # damped-Jacobi spectral radius (1 + 4 cos(pi/5)) / 5 = 0.847 -> ~85 sweeps
+    assert 60 <= n <= 110
+    assert O.u128_to_double(s_last) <= 1e-12 * O.u128_to_double(s0)
+    g2, _ = O.jacobi_run(4, 4, n, seed=2)
+    assert np.array_equal(g, g2)
+    _, n_cap, _, _ = O.jacobi_solve_tol(64, 64, 50, 1e-6, seed=2)
+    assert n_cap == 50
