@@ -31,9 +31,19 @@ DEVICE_FLAGS = [
 
 # (library name, sources, extra link libraries from this build)
 LIBS = [
+    ("libcharming_amd.so", ["runtime/host_runtime.hip"], []),
     ("libchm_jacobi2d.so", ["jacobi2d/plain.hip", "jacobi2d/persistent_stub.cpp"],
-     []),
+     ["charming_amd"]),
 ]
+
+# chare-program executables: the program's TU + the library-provided main
+EXES = [
+    ("hello", ["examples/hello.hip"]),
+    ("refnum", ["examples/refnum.hip"]),
+    ("pingpong", ["examples/pingpong.hip"]),
+]
+MAIN_SRC = "runtime/main.cpp"
+BINDIR = os.path.join(PKG, "bin")
 
 
 def _obj(src):
@@ -65,10 +75,23 @@ def _link(name, srcs, deps, verbose):
         raise RuntimeError(f"link failed for {name}:\n{p.stdout}{p.stderr}")
 
 
+def _link_exe(name, srcs, verbose):
+    cmd = [HIPCC, "--offload-arch=gfx950", "-o", os.path.join(BINDIR, name)]
+    cmd += [_obj(s) for s in srcs] + [_obj(MAIN_SRC)]
+    cmd += ["-L", LIBDIR, "-lcharming_amd", "-Wl,-rpath,$ORIGIN/../lib", "-lpthread"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError(f"link failed for {name}:\n{p.stdout}{p.stderr}")
+
+
 def build(verbose=False, jobs=None):
     os.makedirs(LIBDIR, exist_ok=True)
     os.makedirs(OBJDIR, exist_ok=True)
-    srcs = sorted({s for _, ss, _ in LIBS for s in ss})
+    os.makedirs(BINDIR, exist_ok=True)
+    srcs = sorted({s for _, ss, _ in LIBS for s in ss} |
+                  {s for _, ss in EXES for s in ss} | {MAIN_SRC})
     missing = [s for s in srcs if not os.path.exists(os.path.join(CSRC, s))]
     if missing:
         raise FileNotFoundError(f"missing sources: {missing}")
@@ -78,6 +101,8 @@ def build(verbose=False, jobs=None):
             f.result()
     for name, ss, deps in LIBS:  # in order: the runtime first
         _link(name, ss, deps, verbose)
+    for name, ss in EXES:
+        _link_exe(name, ss, verbose)
     return LIBDIR
 
 

@@ -1,0 +1,47 @@
+// Public runtime API of the charm:: interface (reference include/charming.h:
+// 8-37): user hooks main_host/main, end/abort, PE queries, barriers and
+// device_atoi.  Including this header in a chare program's translation unit
+// also instantiates the persistent scheduler kernel there and registers it
+// with the host library (libcharming_amd.so) at load time.
+#ifndef _CHARMING_H_
+#define _CHARMING_H_
+
+#include "chare.h"
+#include "message.h"
+#include "kernel.h"
+
+namespace charm {
+
+// Main function executed on the host before the launch (charming.cu:73).
+// Optional: a program that does not define it gets none.
+__attribute__((weak)) void main_host(int argc, char** argv);
+
+// Main function executed on all PEs (by every thread of each PE's workgroup)
+__device__ void main(int argc, char** argv, size_t* argvs, int pe);
+
+// Runtime functions that can be called by the user
+__device__ void end();
+__device__ void abort();
+
+__device__ int my_pe();
+__device__ int n_pes();
+__device__ int my_pe_node();
+__device__ int n_pes_node();
+__device__ int n_nodes();
+
+// Element-level barrier (requires all PEs to participate)
+__device__ void barrier();
+__device__ void barrier_local();
+
+__device__ int device_atoi(const char* str, int strlen);
+
+// Seconds on the device wall clock (constant rate).
+__device__ double wall_time();
+
+}  // namespace charm
+
+#ifndef CHARMING_NO_RUNTIME
+#include "charming_detail/device_rt.h"
+#endif
+
+#endif  // _CHARMING_H_

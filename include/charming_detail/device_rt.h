@@ -1,0 +1,608 @@
+// Device runtime of charming_amd: the persistent scheduler, the HBM inbox
+// rings, refnum deferral/revival, grid barrier and termination.  Included
+// (once) by <charming.h> in the chare program's translation unit, which
+// therefore holds the scheduler kernel and the user's entry methods in one
+// device code object: no relocatable device code is needed (the reference
+// links its scheduler with nvcc -dc, Makefile:38).
+//
+// One PE = one workgroup of the cooperative launch (the reference's SM_LEVEL
+// mode, src/sched/scheduler.cu:183-234), with no CE workgroup reserved.
+//
+// Inbox protocol (replaces the per-pair slot mailbox + min-heap of
+// src/comm/get/get.cu:209-781 and ringbuf.cu/heap.cu):
+//   producer (thread 0): pos = fetch_add(tail[dst], slot)          (relaxed)
+//                        wait until head[dst] >= pos + slot - R     (relaxed)
+//   all threads:         write envelope + header + payload at pos
+//   each wave:           s_waitcnt 0; workgroup barrier
+//   producer (thread 0): ready[dst][pos/64 mod N] = pos/64 + 1     (release)
+//   consumer (thread 0): poll ready[me][read/64 mod N] == read/64+1 (relaxed)
+//                        then an agent-scope acquire fence
+//   all threads:         run the entry method on the message in place
+//   consumer (thread 0): read += slot; head[me] = read             (relaxed,
+//                        after the reads completed)
+// Tags are logical granule numbers, unique for the life of the rings, so a
+// stale word can never look published; messages are consumed in FIFO order
+// and a message that straddles the ring end continues into the overflow tail.
+#ifndef CHARMING_DETAIL_DEVICE_RT_H_
+#define CHARMING_DETAIL_DEVICE_RT_H_
+
+#include <hip/hip_runtime.h>
+
+#include "charming_amd.h"
+#include "charming_detail/ctx.h"
+
+#ifndef CHARMING_PE_THREADS
+#define CHARMING_PE_THREADS 512  // threads per PE workgroup (charming.cu:208)
+#endif
+
+namespace charm {
+namespace rt {
+
+static __constant__ Ctx g_ctx;
+
+constexpr uint64_t kFail = ~0ull;
+enum : int { CMD_EXIT = 0, CMD_INBOX = 1, CMD_REVIVED = 2 };
+
+// Workgroup-shared scratch used to broadcast thread 0's decisions.
+struct Lds {
+  char* ptr;
+  uint64_t pos;
+  int cmd;
+  int arg;
+  int n;
+  int pad;
+  uint32_t order[MISMATCH_MAX];
+};
+static __shared__ Lds s_lds;
+
+// ---------------------------------------------------------------------------
+// memory-model helpers (agent scope = the whole GPU, all 8 XCDs)
+// ---------------------------------------------------------------------------
+__device__ inline uint64_t ld_relaxed(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint32_t ld_relaxed(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_relaxed(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_release(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void fence_acquire() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+// Wait until every memory operation this wave issued has completed.
+__device__ inline void wave_drain() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+__device__ PeState* pe_state() { return g_ctx.pes + blockIdx.x; }
+__device__ inline Lds& lds() { return s_lds; }
+__device__ inline uint64_t n_granules() { return g_ctx.ring_bytes / kGranule; }
+
+__device__ void raise_error(uint32_t code, uint32_t aux) {
+  uint32_t* ctl = g_ctx.ctl;
+  uint32_t expected = 0;
+  if (__hip_atomic_compare_exchange_strong(&ctl[CTL_ERROR], &expected, code,
+                                           __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT)) {
+    __hip_atomic_store(&ctl[CTL_ERROR_PE], (uint32_t)blockIdx.x,
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&ctl[CTL_ERROR_AUX], aux, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __hip_atomic_store(&ctl[CTL_TERM], 1u, __ATOMIC_RELEASE,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline bool stopping() {
+  return ld_relaxed(&g_ctx.ctl[CTL_TERM]) != 0 || *g_ctx.host_abort != 0;
+}
+
+// Bump allocator over the PE's heap slice (single-threaded).
+__device__ void* heap_alloc(size_t bytes, size_t align) {
+  PeState* ps = pe_state();
+  uint64_t off = (ps->heap_used + align - 1) / align * align;
+  if (off + bytes > g_ctx.heap_bytes) {
+    raise_error(ERR_HEAP_FULL, (uint32_t)bytes);
+    return nullptr;
+  }
+  ps->heap_used = off + bytes;
+  return g_ctx.heap + (uint64_t)blockIdx.x * g_ctx.heap_bytes + off;
+}
+
+__device__ inline void* user_ptr() { return g_ctx.user; }
+
+// ---------------------------------------------------------------------------
+// workgroup copy: 16-byte words when the source allows it, 4 in flight per
+// thread, then an alignment-tiered tail.  dst is 16-byte aligned.
+// ---------------------------------------------------------------------------
+__device__ void wg_copy(char* dst, const char* src, size_t n) {
+  const size_t t = threadIdx.x, T = blockDim.x;
+  const uintptr_t sa = (uintptr_t)src;
+  if ((sa & 15) == 0) {
+    const size_t n16 = n >> 4;
+    const uint4* s = (const uint4*)src;
+    uint4* d = (uint4*)dst;
+    size_t i = t;
+    for (; i + 3 * T < n16; i += 4 * T) {
+      uint4 a = s[i], b = s[i + T], c = s[i + 2 * T], e = s[i + 3 * T];
+      d[i] = a;
+      d[i + T] = b;
+      d[i + 2 * T] = c;
+      d[i + 3 * T] = e;
+    }
+    for (; i < n16; i += T) d[i] = s[i];
+    for (size_t k = (n16 << 4) + t; k < n; k += T) dst[k] = src[k];
+  } else if ((sa & 7) == 0) {
+    const size_t n8 = n >> 3;
+    for (size_t i = t; i < n8; i += T)
+      ((uint64_t*)dst)[i] = ((const uint64_t*)src)[i];
+    for (size_t k = (n8 << 3) + t; k < n; k += T) dst[k] = src[k];
+  } else if ((sa & 3) == 0) {
+    const size_t n4 = n >> 2;
+    for (size_t i = t; i < n4; i += T)
+      ((uint32_t*)dst)[i] = ((const uint32_t*)src)[i];
+    for (size_t k = (n4 << 2) + t; k < n; k += T) dst[k] = src[k];
+  } else {
+    for (size_t k = t; k < n; k += T) dst[k] = src[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// send
+// ---------------------------------------------------------------------------
+
+// Reserve `slot` bytes in dst's ring (thread 0).  Returns kFail when the
+// runtime is stopping or the request is invalid.
+__device__ uint64_t reserve(int dst, uint64_t slot) {
+  const Ctx& c = g_ctx;
+  if (slot > c.max_msg_bytes) {
+    raise_error(ERR_MSG_TOO_LARGE, (uint32_t)(slot >> 4));
+    return kFail;
+  }
+  if (dst < 0 || dst >= c.n_pes) {
+    raise_error(ERR_BAD_CHARE, (uint32_t)dst);
+    return kFail;
+  }
+  const uint64_t pos = __hip_atomic_fetch_add(&c.tails[(uint64_t)dst * 16],
+                                              slot, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+  const uint64_t head0 = ld_relaxed(&c.heads[(uint64_t)dst * 16]);
+  if (pos + slot - head0 > c.ring_bytes) {
+    const uint64_t need = pos + slot - c.ring_bytes;
+    uint64_t spins = 0;
+    while (ld_relaxed(&c.heads[(uint64_t)dst * 16]) < need) {
+      if (dst == (int)blockIdx.x) {
+        raise_error(ERR_SELF_RING_FULL, (uint32_t)slot);
+        return kFail;
+      }
+      if (stopping()) return kFail;
+      if (c.spin_limit && ++spins > c.spin_limit) {
+        raise_error(ERR_SPIN_LIMIT, (uint32_t)dst);
+        return kFail;
+      }
+      pe_state()->stats.send_spins++;
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+  return pos;
+}
+
+// Workgroup-collective send of one message to PE dst.
+__device__ void send_msg(int dst, msgtype type, int chare_id, int chare_idx,
+                         int ep, int refnum, const void* buf, size_t payload) {
+  const Ctx& c = g_ctx;
+  Lds& L = lds();
+  const uint64_t alloc = envelope::alloc_size(type, payload);
+  const uint64_t slot = (alloc + kGranule - 1) & ~(kGranule - 1);
+  __syncthreads();
+  if (threadIdx.x == 0) L.pos = reserve(dst, slot);
+  __syncthreads();
+  const uint64_t pos = L.pos;
+  if (pos == kFail) return;
+  char* base = c.rings + (uint64_t)dst * c.ring_stride +
+               (pos & (c.ring_bytes - 1));
+  if (threadIdx.x == 0) {
+    new (base) envelope(type, alloc);
+    new (base + sizeof(envelope))
+        regular_msg(chare_id, chare_idx, ep, refnum);
+  }
+  if (payload) wg_copy(base + sizeof(envelope) + sizeof(regular_msg),
+                       (const char*)buf, payload);
+  wave_drain();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint64_t g = pos / kGranule;
+    st_release(&c.ready[(uint64_t)dst * n_granules() + (g & (n_granules() - 1))],
+               g + 1);
+    pe_state()->stats.sent++;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// deferral store (refnum mismatches), owner-only
+// ---------------------------------------------------------------------------
+__device__ inline char* arena() {
+  return g_ctx.deferred + (uint64_t)blockIdx.x * g_ctx.deferred_bytes;
+}
+
+// Move [src, src+bytes) down to dst < src, in workgroup-wide chunks whose
+// loads all finish before any of their stores.
+__device__ void move_down(char* dst, const char* src, uint32_t bytes) {
+  const uint32_t n16 = bytes >> 4;
+  for (uint32_t base = 0; base < n16; base += blockDim.x) {
+    const uint32_t i = base + threadIdx.x;
+    uint4 v;
+    if (i < n16) v = ((const uint4*)src)[i];
+    __syncthreads();
+    if (i < n16) ((uint4*)dst)[i] = v;
+    __syncthreads();
+  }
+}
+
+// Collective: pack live deferred messages to the start of the arena.
+__device__ void compact() {
+  PeState* ps = pe_state();
+  Lds& L = lds();
+  if (threadIdx.x == 0) {
+    int n = 0;
+    for (int i = 0; i < MISMATCH_MAX; i++)
+      if (ps->deferred[i].state != 0) {
+        int k = n++;  // insertion sort by offset
+        while (k > 0 && ps->deferred[L.order[k - 1]].offset >
+                            ps->deferred[i].offset) {
+          L.order[k] = L.order[k - 1];
+          k--;
+        }
+        L.order[k] = (uint32_t)i;
+      }
+    L.n = n;
+  }
+  __syncthreads();
+  const int n = L.n;
+  uint32_t off = 0;
+  for (int k = 0; k < n; k++) {
+    DeferredEntry& d = ps->deferred[L.order[k]];
+    const uint32_t src = d.offset, bytes = (d.bytes + 15) & ~15u;
+    if (src != off) move_down(arena() + off, arena() + src, bytes);
+    __syncthreads();
+    if (threadIdx.x == 0) d.offset = off;
+    off += bytes;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) ps->arena_used = off;
+  __syncthreads();
+}
+
+// Collective: copy the message at msg into the deferral store.
+__device__ void defer(const char* msg, uint64_t alloc, int chare_id,
+                      int chare_idx, int refnum) {
+  PeState* ps = pe_state();
+  Lds& L = lds();
+  const uint32_t need = (uint32_t)((alloc + 15) & ~15ull);
+  if (threadIdx.x == 0)
+    L.arg = (ps->arena_used + (uint64_t)need > g_ctx.deferred_bytes) ? 1 : 0;
+  __syncthreads();
+  if (L.arg) compact();
+  if (threadIdx.x == 0) {
+    int e = -1;
+    for (int i = 0; i < MISMATCH_MAX; i++)
+      if (ps->deferred[i].state == 0) {
+        e = i;
+        break;
+      }
+    if (e < 0 || ps->arena_used + (uint64_t)need > g_ctx.deferred_bytes) {
+      raise_error(ERR_DEFERRED_FULL, (uint32_t)chare_idx);
+      L.arg = -1;
+    } else {
+      DeferredEntry& d = ps->deferred[e];
+      d.offset = ps->arena_used;
+      d.bytes = (uint32_t)alloc;
+      d.chare_id = chare_id;
+      d.chare_idx = chare_idx;
+      d.refnum = refnum;
+      d.state = 1;
+      d.seq = ps->seq++;
+      ps->arena_used += need;
+      ps->n_deferred++;
+      ps->stats.deferred++;
+      L.arg = e;
+      L.ptr = arena() + d.offset;
+    }
+  }
+  __syncthreads();
+  if (L.arg >= 0) wg_copy(L.ptr, msg, alloc);
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// dispatch and the scheduler loop
+// ---------------------------------------------------------------------------
+
+// Collective.  Decodes the envelope (scheduler.cu:23-76) and runs the entry
+// method through the refnum gate; false = must be deferred.
+__device__ bool dispatch(envelope* env) {
+  if (env->type == msgtype::regular || env->type == msgtype::user) {
+    regular_msg* m = (regular_msg*)((char*)env + sizeof(envelope));
+    ProxyTable& t = pe_state()->table;
+    const int id = m->chare_id;
+    if (id < 0 || id >= t.count) {
+      if (threadIdx.x == 0) raise_error(ERR_BAD_MESSAGE, (uint32_t)id);
+      return true;
+    }
+    return t.proxies[id]->call(m->chare_idx, m->ep_id,
+                               (char*)m + sizeof(regular_msg), m->refnum);
+  }
+  if (threadIdx.x == 0) raise_error(ERR_BAD_MESSAGE, (uint32_t)env->type);
+  return true;
+}
+
+// Thread 0: wait for work.  Revived messages go first (in arrival order),
+// then the inbox in FIFO order.
+__device__ int poll_next() {
+  const Ctx& c = g_ctx;
+  PeState* ps = pe_state();
+  Lds& L = lds();
+  const uint64_t NG = n_granules();
+  const uint64_t* rdy = c.ready + (uint64_t)blockIdx.x * NG;
+  char* ring = c.rings + (uint64_t)blockIdx.x * c.ring_stride;
+  for (uint32_t spins = 0;; spins++) {
+    if (ps->n_revived > 0) {
+      int best = -1;
+      for (int i = 0; i < MISMATCH_MAX; i++)
+        if (ps->deferred[i].state == 2 &&
+            (best < 0 || ps->deferred[i].seq < ps->deferred[best].seq))
+          best = i;
+      if (best >= 0) {
+        ps->deferred[best].state = 3;  // being processed
+        ps->n_revived--;
+        L.arg = best;
+        L.ptr = arena() + ps->deferred[best].offset;
+        return CMD_REVIVED;
+      }
+      ps->n_revived = 0;
+    }
+    const uint64_t g = ps->read_pos / kGranule;
+    if (ld_relaxed(&rdy[g & (NG - 1)]) == g + 1) {
+      fence_acquire();
+      L.ptr = ring + (ps->read_pos & (c.ring_bytes - 1));
+      return CMD_INBOX;
+    }
+    if (ld_relaxed(&c.ctl[CTL_TERM]) != 0) return CMD_EXIT;
+    if ((spins & 255) == 255) {
+      if (*c.host_abort) {
+        raise_error(ERR_HOST_ABORT, 0);
+        return CMD_EXIT;
+      }
+      ps->stats.idle_polls++;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+__device__ void process_inbox() {
+  const Ctx& c = g_ctx;
+  PeState* ps = pe_state();
+  envelope* env = (envelope*)lds().ptr;
+  const uint64_t alloc = env->size;
+  const uint64_t slot = (alloc + kGranule - 1) & ~(kGranule - 1);
+  bool ok = true;
+  if (alloc < sizeof(envelope) || slot > c.max_msg_bytes) {
+    if (threadIdx.x == 0) raise_error(ERR_BAD_MESSAGE, (uint32_t)alloc);
+  } else {
+    ok = dispatch(env);
+    if (!ok) {
+      regular_msg* m = (regular_msg*)((char*)env + sizeof(envelope));
+      defer((const char*)env, alloc, m->chare_id, m->chare_idx, m->refnum);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    wave_drain();
+    ps->read_pos += slot;
+    st_relaxed(&c.heads[(uint64_t)blockIdx.x * 16], ps->read_pos);
+    if (ok) ps->stats.delivered++;
+  }
+}
+
+__device__ void process_revived() {
+  PeState* ps = pe_state();
+  const int e = lds().arg;
+  envelope* env = (envelope*)lds().ptr;
+  const bool ok = dispatch(env);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    DeferredEntry& d = ps->deferred[e];
+    if (ok) {
+      d.state = 0;
+      ps->n_deferred--;
+      ps->stats.delivered++;
+      ps->stats.revived++;
+      if (ps->n_deferred == 0) ps->arena_used = 0;
+    } else {
+      d.state = 1;  // refnum moved on again: stay deferred
+    }
+  }
+}
+
+__device__ void pe_loop() {
+  Lds& L = lds();
+  for (;;) {
+    if (threadIdx.x == 0) L.cmd = poll_next();
+    __syncthreads();
+    const int cmd = L.cmd;
+    if (cmd == CMD_EXIT) break;
+    if (cmd == CMD_INBOX)
+      process_inbox();
+    else
+      process_revived();
+    __syncthreads();
+  }
+}
+
+// Collective grid barrier over all PEs (generation counter).  Replaces
+// nvshmem_barrier_all + grid.sync (scheduler.cu:336-350).
+__device__ void grid_barrier() {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t* cnt = &g_ctx.ctl[CTL_BAR_COUNT];
+    uint32_t* gen = &g_ctx.ctl[CTL_BAR_GEN];
+    const uint32_t g = ld_relaxed(gen);
+    const uint32_t arrived = __hip_atomic_fetch_add(
+        cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (arrived == (uint32_t)g_ctx.n_pes - 1) {
+      __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(gen, g + 1, __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      for (uint32_t spins = 0; ld_relaxed(gen) == g; spins++) {
+        if ((spins & 255) == 255 && *g_ctx.host_abort) {
+          raise_error(ERR_HOST_ABORT, 1);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      fence_acquire();
+    }
+  }
+  __syncthreads();
+}
+
+// Thread 0 at kernel start: empty the inbox (skip anything a previous run
+// left in flight), reset owner state.
+__device__ void pe_begin() {
+  if (threadIdx.x == 0) {
+    PeState* ps = pe_state();
+    const uint64_t t = ld_relaxed(&g_ctx.tails[(uint64_t)blockIdx.x * 16]);
+    ps->read_pos = t;
+    st_relaxed(&g_ctx.heads[(uint64_t)blockIdx.x * 16], t);
+    ps->table.count = 0;
+    ps->heap_used = 0;
+    ps->arena_used = 0;
+    ps->seq = 0;
+    ps->n_deferred = 0;
+    ps->n_revived = 0;
+    ps->ended = 0;
+    for (int i = 0; i < MISMATCH_MAX; i++) ps->deferred[i].state = 0;
+    ps->stats = Stats{};
+  }
+  __syncthreads();
+}
+
+}  // namespace rt
+
+// ---------------------------------------------------------------------------
+// charm:: runtime functions (charming.h, message.h)
+// ---------------------------------------------------------------------------
+__device__ void send_reg_msg(int chare_id, int chare_idx, int ep_id, void* buf,
+                             size_t payload_size, int dst_pe, int refnum) {
+  rt::send_msg(dst_pe, msgtype::regular, chare_id, chare_idx, ep_id, refnum,
+               buf, payload_size);
+}
+
+// Single-threaded (get.cu:1316-1345): deferred messages of (chare, refnum)
+// become eligible again; the scheduler re-dispatches them before new inbox
+// messages, in arrival order.
+__device__ void revive_mismatches(int chare_id, int chare_idx, int refnum) {
+  rt::PeState* ps = rt::pe_state();
+  if (ps->n_deferred == 0) return;
+  for (int i = 0; i < MISMATCH_MAX; i++) {
+    rt::DeferredEntry& d = ps->deferred[i];
+    if (d.state == 1 && d.chare_id == chare_id && d.chare_idx == chare_idx &&
+        d.refnum == refnum) {
+      d.state = 2;
+      ps->n_revived++;
+    }
+  }
+}
+
+// charm::end() (charming.cu:265-267): the first call anywhere terminates
+// every PE's scheduler loop; messages still in flight are dropped, as in
+// the reference (no quiescence detection).  Any thread may call it.
+__device__ void send_begin_term_msg() {
+  __hip_atomic_store(&rt::g_ctx.ctl[rt::CTL_TERM], 1u, __ATOMIC_RELEASE,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ void end() {
+  if (threadIdx.x == 0) rt::pe_state()->ended = 1;
+  send_begin_term_msg();
+}
+
+// charm::abort(): stops every PE and records ERR_USER_ABORT.  (The
+// reference stops only the calling PE, charming.cu:269-278, which leaves the
+// others waiting at the final barrier.)
+__device__ void abort() { rt::raise_error(rt::ERR_USER_ABORT, blockIdx.x); }
+
+__device__ int my_pe() { return (int)blockIdx.x; }
+__device__ int n_pes() { return rt::g_ctx.n_pes; }
+__device__ int my_pe_node() { return (int)blockIdx.x; }
+__device__ int n_pes_node() { return rt::g_ctx.n_pes; }
+__device__ int n_nodes() { return 1; }
+
+__device__ void barrier() { rt::grid_barrier(); }
+__device__ void barrier_local() { rt::grid_barrier(); }
+
+// charming.cu:294-304
+__device__ int device_atoi(const char* str, int strlen) {
+  int tmp = 0;
+  for (int i = 0; i < strlen; i++) {
+    int multiplier = 1;
+    for (int j = 0; j < strlen - i - 1; j++) multiplier *= 10;
+    tmp += (str[i] - 48) * multiplier;
+  }
+  return tmp;
+}
+
+__device__ double wall_time() {
+  return (double)wall_clock64() / rt::g_ctx.wall_hz;
+}
+
+}  // namespace charm
+
+// ---------------------------------------------------------------------------
+// the persistent scheduler kernel (scheduler.cu:183-234) and its registration
+// ---------------------------------------------------------------------------
+namespace charm {
+namespace rt {
+
+template <int kThreads>
+__global__ void __launch_bounds__(kThreads) scheduler_kernel(int unused) {
+  (void)unused;
+  pe_begin();
+  grid_barrier();
+  charm::main(g_ctx.argc, g_ctx.argv, g_ctx.argvs, (int)blockIdx.x);
+  grid_barrier();
+  pe_loop();
+  grid_barrier();
+}
+
+static int set_ctx_host(const void* ctx, size_t bytes) {
+  if (bytes != sizeof(Ctx)) return -1;
+  return hipMemcpyToSymbol(HIP_SYMBOL(g_ctx), ctx, sizeof(Ctx)) == hipSuccess
+             ? 0
+             : -1;
+}
+
+struct ProgramRegistrar {
+  ProgramRegistrar() {
+    chm_program p;
+    p.kernel = (const void*)&scheduler_kernel<CHARMING_PE_THREADS>;
+    p.threads_per_pe = CHARMING_PE_THREADS;
+    p.main_host = &charm::main_host;
+    p.set_ctx = &set_ctx_host;
+    p.name = __FILE__;
+    chm_register_program(&p);
+  }
+};
+static ProgramRegistrar s_program_registrar;
+
+}  // namespace rt
+}  // namespace charm
+
+#endif  // CHARMING_DETAIL_DEVICE_RT_H_
