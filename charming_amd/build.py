@@ -32,7 +32,7 @@ DEVICE_FLAGS = [
 # (library name, sources, extra link libraries from this build)
 LIBS = [
     ("libcharming_amd.so", ["runtime/host_runtime.hip"], []),
-    ("libchm_jacobi2d.so", ["jacobi2d/plain.hip", "jacobi2d/persistent_stub.cpp"],
+    ("libchm_jacobi2d.so", ["jacobi2d/plain.hip", "jacobi2d/persistent.hip"],
      ["charming_amd"]),
 ]
 

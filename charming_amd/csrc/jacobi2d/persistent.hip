@@ -1,0 +1,550 @@
+// Message-driven Jacobi2D on the persistent runtime: the chare program of
+// examples/jacobi2d/jacobi2d.cu:26-335 written against the current charm::
+// API, plus the host side of the persistent form of the Jacobi2D C ABI.
+//
+// Differences from the reference program, each deliberate:
+//   - halos carry refnum = iteration and the chare's refnum advances with
+//     its iteration, so a neighbour one iteration ahead is deferred instead
+//     of being miscounted (the reference counts halos without refnums,
+//     jacobi2d.cu:222);
+//   - the stencil, pack and unpack run inside the PE's workgroup (HIP has no
+//     device-side launch, which the reference uses at jacobi2d.cu:145-235);
+//     the sweep writes the four outgoing edges straight into the staging
+//     buffers the next sends copy from (pack fused into the sweep);
+//   - grids are allocated once by the host in HBM and seeded by a kernel
+//     (no device `new`, jacobi2d.cu:132-141);
+//   - the chare-column count divides the width (the reference divides the
+//     width by the chare-row count, jacobi2d.cu:64-65, a transposition that
+//     is harmless only for square chare grids);
+//   - the last row/column of chares takes the remainder of an uneven split;
+//   - a chare with no neighbours iterates without waiting.
+#define CHARMING_PE_THREADS 1024
+#include <charming.h>
+
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "jacobi2d_internal.h"
+
+#define LEFT 0
+#define RIGHT 1
+#define TOP 2
+#define BOTTOM 3
+#define N_NEIGHBORS 4
+
+namespace chm {
+namespace jac {
+
+// Problem description shared with the device (one per solve).
+struct JacobiDesc {
+  double* arena;          // chare c: arrays at arena + (2c + k) * chare_elems
+  int64_t chare_elems;
+  double* staging;        // chare c: 4 edges at staging + (4c + d) * stage_elems
+  int64_t stage_elems;
+  uint64_t* res_acc;      // exact residual of the last sweep {lo, hi}
+  int W, H, cr, cc, iters;
+  int tune_nc, tune_u;
+};
+
+__device__ JacobiDesc g_jdesc;
+
+__host__ __device__ inline ChareGeom chare_geom(int W, int H, int cr, int cc,
+                                               int idx) {
+  ChareGeom g;
+  split_extent(H, cr, idx / cc, &g.row0, &g.bh);
+  split_extent(W, cc, idx % cc, &g.col0, &g.bw);
+  return g;
+}
+
+}  // namespace jac
+}  // namespace chm
+
+using namespace chm::jac;
+
+// ---------------------------------------------------------------------------
+// the chare program
+// ---------------------------------------------------------------------------
+struct Block : charm::chare {
+  int row, col;
+  int neighbor_index[N_NEIGHBORS];
+  int neighbor_count;
+  int recv_count;
+  int end_count;
+  int block_width, block_height;
+  int64_t pitch;
+  int n_iters, iter;
+  double* grids[2];      // u_k lives in grids[k & 1]
+  double* boundaries[N_NEIGHBORS];  // [dir slot | edge] staging
+
+  __device__ Block() {}
+  __device__ void init(void* arg);
+  __device__ void send_boundaries();
+  __device__ void recv_ghosts(void* arg);
+  __device__ void update();
+  __device__ void end(void* arg);
+};
+
+__device__ void entry_init(Block& b, void* arg) { b.init(arg); }
+__device__ void entry_recv_ghosts(Block& b, void* arg) { b.recv_ghosts(arg); }
+__device__ void entry_end(Block& b, void* arg) { b.end(arg); }
+
+__shared__ charm::chare_proxy<Block>* block_proxy;
+
+__device__ void charm::main(int argc, char** argv, size_t* argvs, int pe) {
+  const JacobiDesc& d = g_jdesc;
+  if (threadIdx.x == 0) {
+    void* mem = charm::rt::heap_alloc(sizeof(charm::chare_proxy<Block>), 16);
+    block_proxy = new (mem) charm::chare_proxy<Block>();
+    block_proxy->add_entry_method<&entry_init>();
+    block_proxy->add_entry_method<&entry_recv_ghosts>();
+    block_proxy->add_entry_method<&entry_end>();
+    block_proxy->create(d.cr * d.cc);
+  }
+  __syncthreads();
+  if (pe == 0) {
+    __shared__ int params[5];
+    if (threadIdx.x == 0) {
+      params[0] = d.W;
+      params[1] = d.H;
+      params[2] = d.iters;
+      params[3] = d.cr;
+      params[4] = d.cc;
+    }
+    __syncthreads();
+    block_proxy->invoke_all(0, params, sizeof(params));
+  }
+}
+
+// The sweep of one chare by its PE's workgroup: waves take (panel, band)
+// tiles of the block.  RES accumulates the exact residual of this sweep.
+template <int NC, int U>
+__device__ void chare_sweep(const double* in, double* out, int64_t P, int bw,
+                            int bh, bool res, const EdgeOut& eo) {
+  const int wave = threadIdx.x / 64, W = blockDim.x / 64;
+  const int pw = 128 * NC;
+  const int n_panels = (bw + pw - 1) / pw;
+  int n_bands = W / n_panels;
+  if (n_bands < 1) n_bands = 1;
+  if (n_bands > bh) n_bands = bh;
+  const int band_rows = (bh + n_bands - 1) / n_bands;
+  const int tiles = n_panels * n_bands;
+  u128acc acc = {0, 0};
+  for (int t = wave; t < tiles; t += W) {
+    const int panel = t % n_panels, band = t / n_panels;
+    const int r0 = 1 + band * band_rows;
+    const int r1 = min(r0 + band_rows, bh + 1);
+    const int c0 = 1 + panel * pw;
+    if (res)
+      sweep_panel<NC, U, true, true>(in, out, P, bw, bh, c0, r0, r1, acc, eo);
+    else
+      sweep_panel<NC, U, false, true>(in, out, P, bw, bh, c0, r0, r1, acc, eo);
+  }
+  if (res) {
+    acc = wave_reduce_u128(acc);
+    if (__lane_id() == 0) atomic_add_u128(g_jdesc.res_acc, acc);
+  }
+}
+
+__device__ void sweep_dispatch(const double* in, double* out, int64_t P,
+                               int bw, int bh, bool res, const EdgeOut& eo) {
+  int nc = g_jdesc.tune_nc, u = g_jdesc.tune_u;
+  // Entry methods are reached through a function pointer, so their code is
+  // limited to 128 VGPRs whatever the launch bounds: only tiles whose row
+  // ring fits (NC * U <= 4) are instantiated; with 16 waves per PE that is
+  // 16 x 4 KiB of reads in flight per CU.
+  if (nc == 0) {
+    nc = bw > 256 ? 2 : 1;
+    u = 4 / nc;
+  }
+  if (u == 0) u = (4 / nc) > 0 ? 4 / nc : 1;
+#define CHM_CASE(NC_, U_)                                   \
+  if (nc == NC_ && u == U_) {                               \
+    chare_sweep<NC_, U_>(in, out, P, bw, bh, res, eo);      \
+    return;                                                 \
+  }
+  CHM_CASE(1, 4) CHM_CASE(2, 2) CHM_CASE(4, 1) CHM_CASE(1, 2) CHM_CASE(2, 1)
+  CHM_CASE(1, 1)
+#undef CHM_CASE
+  chare_sweep<1, 4>(in, out, P, bw, bh, res, eo);
+}
+
+__device__ void Block::init(void* arg) {
+  const JacobiDesc& d = g_jdesc;
+  if (threadIdx.x == 0) {
+    const int* params = (const int*)arg;
+    n_iters = params[2];
+    const int cr = params[3], cc = params[4];
+    const int index = charm::chare::i;
+    const ChareGeom g = chare_geom(params[0], params[1], cr, cc, index);
+    block_width = g.bw;
+    block_height = g.bh;
+    pitch = pitch_for(g.bw);
+    iter = 0;
+    row = index / cc;
+    col = index % cc;
+    neighbor_index[LEFT] = (col == 0) ? -1 : index - 1;
+    neighbor_index[RIGHT] = (col == cc - 1) ? -1 : index + 1;
+    neighbor_index[TOP] = (row == 0) ? -1 : index - cc;
+    neighbor_index[BOTTOM] = (row == cr - 1) ? -1 : index + cc;
+    neighbor_count = 0;
+    for (int k = 0; k < N_NEIGHBORS; k++)
+      neighbor_count += (neighbor_index[k] >= 0);
+    recv_count = 0;
+    end_count = 0;
+    grids[0] = d.arena + (int64_t)(2 * index) * d.chare_elems;
+    grids[1] = grids[0] + d.chare_elems;
+    for (int k = 0; k < N_NEIGHBORS; k++) {
+      boundaries[k] = d.staging + (int64_t)(4 * index + k) * d.stage_elems;
+      // dir tag = the receiver's side (jacobi2d.cu:139-140)
+      ((int*)boundaries[k])[0] = (k % 2 == 0) ? k + 1 : k - 1;
+      ((int*)boundaries[k])[1] = 0;
+    }
+    block_proxy->set_refnum(index, 0);
+  }
+  __syncthreads();
+  // pack the edges of u_0 (later iterations pack inside the sweep)
+  const double* t = grids[0];
+  const int bw = block_width, bh = block_height;
+  const int64_t P = pitch;
+  for (int k = threadIdx.x; k < bw; k += blockDim.x) {
+    boundaries[TOP][1 + k] = t[P + kColOff + 1 + k];
+    boundaries[BOTTOM][1 + k] = t[P * bh + kColOff + 1 + k];
+  }
+  for (int k = threadIdx.x; k < bh; k += blockDim.x) {
+    boundaries[LEFT][1 + k] = t[P * (1 + k) + kColOff + 1];
+    boundaries[RIGHT][1 + k] = t[P * (1 + k) + kColOff + bw];
+  }
+  __syncthreads();
+  send_boundaries();
+  if (neighbor_count == 0) update();
+}
+
+__device__ void Block::send_boundaries() {
+  // (jacobi2d.cu:152-194) four sends tagged with the iteration
+  const int it = iter;
+  for (int k = 0; k < N_NEIGHBORS; k++) {
+    if (neighbor_index[k] < 0) continue;
+    const int len = (k == LEFT || k == RIGHT) ? block_height : block_width;
+    block_proxy->invoke(neighbor_index[k], 1, boundaries[k],
+                        sizeof(double) + sizeof(double) * (size_t)len, it);
+  }
+}
+
+__device__ void Block::recv_ghosts(void* arg) {
+  // (jacobi2d.cu:196-229) unpack into the halo of u_iter
+  const int dir = *(const int*)arg;
+  const double* ghost = (const double*)arg + 1;
+  double* t = grids[iter & 1];
+  const int bw = block_width, bh = block_height;
+  const int64_t P = pitch;
+  switch (dir) {
+    case LEFT:
+      for (int k = threadIdx.x; k < bh; k += blockDim.x)
+        t[P * (1 + k) + kColOff] = ghost[k];
+      break;
+    case RIGHT:
+      for (int k = threadIdx.x; k < bh; k += blockDim.x)
+        t[P * (1 + k) + kColOff + bw + 1] = ghost[k];
+      break;
+    case TOP:
+      for (int k = threadIdx.x; k < bw; k += blockDim.x)
+        t[kColOff + 1 + k] = ghost[k];
+      break;
+    case BOTTOM:
+      for (int k = threadIdx.x; k < bw; k += blockDim.x)
+        t[P * (bh + 1) + kColOff + 1 + k] = ghost[k];
+      break;
+    default:
+      if (threadIdx.x == 0) charm::rt::raise_error(charm::rt::ERR_BAD_MESSAGE, dir);
+      break;
+  }
+  __shared__ int go;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    go = (++recv_count == neighbor_count);
+    if (go) recv_count = 0;
+  }
+  __syncthreads();
+  if (go) update();
+}
+
+__device__ void Block::update() {
+  const int bw = block_width, bh = block_height;
+  const int64_t P = pitch;
+  for (;;) {
+    const bool last = (iter == n_iters - 1);
+    double* t = grids[iter & 1];
+    double* nt = grids[(iter + 1) & 1];
+    EdgeOut eo;
+    eo.top = (neighbor_index[TOP] >= 0) ? boundaries[TOP] + 1 : nullptr;
+    eo.bottom = (neighbor_index[BOTTOM] >= 0) ? boundaries[BOTTOM] + 1 : nullptr;
+    eo.left = (neighbor_index[LEFT] >= 0) ? boundaries[LEFT] + 1 : nullptr;
+    eo.right = (neighbor_index[RIGHT] >= 0) ? boundaries[RIGHT] + 1 : nullptr;
+    sweep_dispatch(t, nt, P, bw, bh, last, eo);
+    __syncthreads();
+    __shared__ int done;
+    if (threadIdx.x == 0) {
+      ++iter;
+      done = (iter == n_iters);
+      if (!done) block_proxy->set_refnum(charm::chare::i, iter);
+    }
+    __syncthreads();
+    if (done) {
+      block_proxy->invoke(0, 2);
+      return;
+    }
+    send_boundaries();
+    if (neighbor_count > 0) return;
+  }
+}
+
+__device__ void Block::end(void* arg) {
+  __shared__ int fin;
+  if (threadIdx.x == 0) fin = (++end_count == charm::chare::n);
+  __syncthreads();
+  if (fin) charm::end();
+}
+
+// ---------------------------------------------------------------------------
+// host side of the persistent form
+// ---------------------------------------------------------------------------
+namespace chm {
+namespace jac {
+
+// u_0 for every chare: interior from the counter-based generator at global
+// coordinates (decomposition invariant), halo ring = ring value, padding 0.
+__global__ void init_chares_kernel(JacobiDesc d, uint64_t seed, int mode,
+                                   double boundary) {
+  const int c = blockIdx.y;
+  const ChareGeom g = chare_geom(d.W, d.H, d.cr, d.cc, c);
+  const int64_t P = pitch_for(g.bw);
+  const int64_t n = (int64_t)(g.bh + 2) * P;
+  double* a = d.arena + (int64_t)(2 * c) * d.chare_elems;
+  double* b = a + d.chare_elems;
+  const double ring = (mode == CHM_INIT_REFERENCE) ? 10.0 : boundary;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / P, j = e % P - kColOff;
+    double v = 0.0;
+    if (j >= 0 && j <= g.bw + 1) {
+      const bool interior = i >= 1 && i <= g.bh && j >= 1 && j <= g.bw;
+      if (!interior)
+        v = ring;
+      else
+        v = (mode == CHM_INIT_REFERENCE)
+                ? 0.0
+                : cell_u01(seed, g.row0 + i, g.col0 + j);
+    }
+    a[e] = v;
+    b[e] = v;
+  }
+}
+
+// Hash of the cells of global rows [r0,r1) x cols [c0,c1) over all chares.
+__global__ void hash_chares_kernel(JacobiDesc d, int parity, int r0, int r1,
+                                   int c0, int c1, uint64_t* out) {
+  const int c = blockIdx.y;
+  const ChareGeom g = chare_geom(d.W, d.H, d.cr, d.cc, c);
+  const int64_t P = pitch_for(g.bw);
+  const double* a = d.arena + (int64_t)(2 * c + parity) * d.chare_elems;
+  const int lr0 = max(r0, g.row0 + 1), lr1 = min(r1, g.row0 + 1 + g.bh);
+  const int lc0 = max(c0, g.col0 + 1), lc1 = min(c1, g.col0 + 1 + g.bw);
+  uint64_t h = 0;
+  if (lr0 < lr1 && lc0 < lc1) {
+    const int64_t cols = lc1 - lc0, n = (int64_t)(lr1 - lr0) * cols;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+      const int64_t gi = lr0 + e / cols, gj = lc0 + e % cols;
+      const int64_t li = gi - g.row0, lj = gj - g.col0;
+      h += hash_cell(gi, gj, a[li * P + kColOff + lj]);
+    }
+  }
+  for (int off = 32; off >= 1; off >>= 1) h += __shfl_down(h, off, 64);
+  if (__lane_id() == 0 && h)
+    __hip_atomic_fetch_add(out, h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct PersistentState {
+  JacobiDesc desc;
+  ChareGeom max_geom;
+  int n_chares;
+  bool own_runtime;
+};
+
+static int g_runtime_users = 0;
+
+static void surface_split(int n, int W, int H, int* cr, int* cc) {
+  // jacobi2d.cu:37-65: minimise the halo surface; rows divide the height
+  double best = 2.0 * ((double)H + (double)W);
+  *cr = 1;
+  *cc = n;
+  for (int ipx = 1; ipx <= n; ipx++) {
+    if (n % ipx) continue;
+    const int ipy = n / ipx;
+    const double surf = 2.0 * ((double)H / ipx + (double)W / ipy);
+    if (surf < best) {
+      best = surf;
+      *cr = ipx;
+      *cc = ipy;
+    }
+  }
+}
+
+static int upload_desc(const JacobiDesc& d) {
+  CHM_TRY_HIP(hipMemcpyToSymbol(HIP_SYMBOL(g_jdesc), &d, sizeof(d)));
+  return CHM_OK;
+}
+
+int persistent_create(Handle* h) {
+  PersistentState* ps = new PersistentState();
+  memset(&ps->desc, 0, sizeof(ps->desc));
+  h->ps = ps;
+  if (g_runtime_users == 0) {
+    CHM_TRY(chm_select_program("persistent.hip") == CHM_OK
+                ? CHM_OK
+                : fail(CHM_E_STATE, "Jacobi2D chare program not registered"));
+    chm_config cfg;
+    memset(&cfg, 0, sizeof(cfg));
+    // watchdog: generous for the largest grids this form is used on
+    // (16384^2 x 1000 sweeps takes about a second)
+    cfg.timeout_s = 120.0;
+    const int rc = chm_init(0, nullptr, &cfg);
+    if (rc != CHM_OK) return fail(rc, "chm_init: %s", chm_last_error());
+  }
+  g_runtime_users++;
+  ps->own_runtime = true;
+  chm_stats st;
+  CHM_TRY(chm_get_stats(&st) == CHM_OK
+              ? CHM_OK
+              : fail(CHM_E_STATE, "runtime stats: %s", chm_last_error()));
+  const int n_pes = st.n_pes;
+  int cr = h->prob.chares_r, cc = h->prob.chares_c;
+  if (cr <= 0 || cc <= 0) {
+    const int n = h->prob.n_chares > 0 ? h->prob.n_chares : n_pes;
+    if (n < 1) return fail(CHM_E_INVALID, "no chares to place (n_pes %d)", n_pes);
+    surface_split(n, h->W, h->H, &cr, &cc);
+  }
+  if (cr < 1 || cc < 1 || cr > h->H || cc > h->W)
+    return fail(CHM_E_INVALID, "%dx%d chares do not fit a %dx%d grid", cr, cc,
+                h->W, h->H);
+  JacobiDesc& d = ps->desc;
+  d.W = h->W;
+  d.H = h->H;
+  d.cr = cr;
+  d.cc = cc;
+  d.iters = h->prob.iters;
+  ps->n_chares = cr * cc;
+  // the last chare row/column is the largest
+  const ChareGeom gmax = chare_geom(h->W, h->H, cr, cc, cr * cc - 1);
+  ps->max_geom = gmax;
+  d.chare_elems = (int64_t)array_elems(gmax.bw, gmax.bh);
+  d.chare_elems = (d.chare_elems + 31) / 32 * 32;  // 256-byte aligned slots
+  d.stage_elems = ((int64_t)std::max(gmax.bw, gmax.bh) + 2 + 15) / 16 * 16;
+  const size_t arena_bytes =
+      (size_t)ps->n_chares * 2 * (size_t)d.chare_elems * sizeof(double);
+  const size_t stage_bytes =
+      (size_t)ps->n_chares * 4 * (size_t)d.stage_elems * sizeof(double);
+  if (hipMalloc(&d.arena, arena_bytes) != hipSuccess ||
+      hipMalloc(&d.staging, stage_bytes) != hipSuccess ||
+      hipMalloc(&d.res_acc, 4 * sizeof(uint64_t)) != hipSuccess)
+    return fail(CHM_E_NOMEM, "persistent form allocation (%zu + %zu bytes)",
+                arena_bytes, stage_bytes);
+  return persistent_reset(h);
+}
+
+int persistent_reset(Handle* h) {
+  PersistentState* ps = h->ps;
+  init_chares_kernel<<<dim3(64, ps->n_chares), 256, 0, h->stream>>>(
+      ps->desc, h->prob.seed, h->prob.init_mode, h->prob.boundary);
+  CHM_TRY_HIP(hipGetLastError());
+  CHM_TRY_HIP(hipStreamSynchronize(h->stream));
+  h->cur = 0;
+  return CHM_OK;
+}
+
+int persistent_solve(Handle* h, chm_jacobi_result* res) {
+  PersistentState* ps = h->ps;
+  JacobiDesc& d = ps->desc;
+  if (h->cur != 0)
+    return fail(CHM_E_STATE, "persistent form restarts from u_0: call reset");
+  d.iters = h->prob.iters;
+  // zero = choose per chare width on the device
+  d.tune_nc = h->tune.panel_chunks;
+  d.tune_u = h->tune.rows_in_flight;
+  CHM_TRY(upload_desc(d));
+  CHM_TRY_HIP(hipMemset(d.res_acc, 0, 4 * sizeof(uint64_t)));
+  CHM_TRY_HIP(hipDeviceSynchronize());
+  if (d.iters == 0) {
+    h->cur = 0;
+  } else {
+    const int rc = chm_run();
+    if (rc != CHM_OK) return fail(rc, "persistent run: %s", chm_last_error());
+    h->cur = d.iters & 1;
+  }
+  if (res) {
+    memset(res, 0, sizeof(*res));
+    chm_stats st;
+    chm_get_stats(&st);
+    res->device_ms = d.iters ? st.kernel_ms : 0.0;
+    res->iters_done = d.iters;
+    res->launches = d.iters ? 1 : 0;
+    res->messages = (long long)st.delivered;
+    res->deferred = (long long)st.deferred;
+    uint64_t acc[2];
+    CHM_TRY_HIP(hipMemcpy(acc, d.res_acc, sizeof(acc), hipMemcpyDeviceToHost));
+    res->residual_lo = acc[0];
+    res->residual_hi = acc[1];
+    CHM_TRY(persistent_hash_region(h, 1, h->H + 1, 1, h->W + 1, &res->hash));
+  }
+  return CHM_OK;
+}
+
+int persistent_hash_region(Handle* h, int r0, int r1, int c0, int c1,
+                           uint64_t* hash) {
+  PersistentState* ps = h->ps;
+  CHM_TRY_HIP(hipMemsetAsync(h->d_acc, 0, sizeof(uint64_t), h->stream));
+  hash_chares_kernel<<<dim3(32, ps->n_chares), 256, 0, h->stream>>>(
+      ps->desc, h->cur, r0, r1, c0, c1, h->d_acc);
+  CHM_TRY_HIP(hipGetLastError());
+  CHM_TRY_HIP(hipMemcpyAsync(hash, h->d_acc, sizeof(uint64_t),
+                             hipMemcpyDeviceToHost, h->stream));
+  CHM_TRY_HIP(hipStreamSynchronize(h->stream));
+  return CHM_OK;
+}
+
+int persistent_read_grid(Handle* h, double* out) {
+  PersistentState* ps = h->ps;
+  const JacobiDesc& d = ps->desc;
+  const int64_t GP = (int64_t)h->W + 2;
+  const double ring =
+      (h->prob.init_mode == CHM_INIT_REFERENCE) ? 10.0 : h->prob.boundary;
+  for (int64_t e = 0; e < GP * (h->H + 2); e++) out[e] = ring;
+  for (int c = 0; c < ps->n_chares; c++) {
+    const ChareGeom g = chare_geom(d.W, d.H, d.cr, d.cc, c);
+    const int64_t P = pitch_for(g.bw);
+    const double* a = d.arena + (int64_t)(2 * c + h->cur) * d.chare_elems;
+    CHM_TRY_HIP(hipMemcpy2DAsync(
+        out + (int64_t)(g.row0 + 1) * GP + g.col0 + 1, GP * sizeof(double),
+        a + P + kColOff + 1, P * sizeof(double), (size_t)g.bw * sizeof(double),
+        (size_t)g.bh, hipMemcpyDeviceToHost, h->stream));
+  }
+  CHM_TRY_HIP(hipStreamSynchronize(h->stream));
+  return CHM_OK;
+}
+
+void persistent_destroy(Handle* h) {
+  PersistentState* ps = h->ps;
+  if (!ps) return;
+  if (ps->desc.arena) (void)hipFree(ps->desc.arena);
+  if (ps->desc.staging) (void)hipFree(ps->desc.staging);
+  if (ps->desc.res_acc) (void)hipFree(ps->desc.res_acc);
+  if (ps->own_runtime && --g_runtime_users == 0) chm_finalize();
+  delete ps;
+  h->ps = nullptr;
+}
+
+}  // namespace jac
+}  // namespace chm

@@ -148,7 +148,12 @@ struct StepLaunch {
 };
 
 static int plan_step(const Handle* h, StepLaunch* L) {
-  const chm_jacobi_tuning& t = h->tune;
+  chm_jacobi_tuning t;
+  default_tuning(&t);
+  if (h->tune.panel_chunks) t.panel_chunks = h->tune.panel_chunks;
+  if (h->tune.rows_in_flight) t.rows_in_flight = h->tune.rows_in_flight;
+  if (h->tune.band_rows) t.band_rows = h->tune.band_rows;
+  if (h->tune.wg_waves) t.wg_waves = h->tune.wg_waves;
   L->fn = pick_step(t.panel_chunks, t.rows_in_flight);
   if (!L->fn)
     return fail(CHM_E_INVALID, "unsupported tuning panel_chunks=%d rows=%d",
@@ -293,7 +298,7 @@ int chm_jacobi2d_create(const chm_jacobi_problem* prob, int form,
   h->W = prob->width;
   h->H = prob->height;
   h->P = pitch_for(h->W);
-  default_tuning(&h->tune);
+  memset(&h->tune, 0, sizeof(h->tune));  // zero fields = per-form defaults
   int rc = CHM_OK;
   do {
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) !=
@@ -337,16 +342,15 @@ int chm_jacobi2d_set_tuning(void* handle, const chm_jacobi_tuning* t) {
   Handle* h = (Handle*)handle;
   if (!h) return fail(CHM_E_INVALID, "null handle");
   chm_jacobi_tuning d;
-  default_tuning(&d);
-  if (t) {
-    if (t->panel_chunks) d.panel_chunks = t->panel_chunks;
-    if (t->rows_in_flight) d.rows_in_flight = t->rows_in_flight;
-    if (t->band_rows) d.band_rows = t->band_rows;
-    if (t->wg_waves) d.wg_waves = t->wg_waves;
-  }
-  if (!pick_step(d.panel_chunks, d.rows_in_flight))
-    return fail(CHM_E_INVALID, "unsupported tuning %d/%d", d.panel_chunks,
-                d.rows_in_flight);
+  memset(&d, 0, sizeof(d));
+  if (t) d = *t;
+  chm_jacobi_tuning chk;
+  default_tuning(&chk);
+  if (d.panel_chunks) chk.panel_chunks = d.panel_chunks;
+  if (d.rows_in_flight) chk.rows_in_flight = d.rows_in_flight;
+  if (!pick_step(chk.panel_chunks, chk.rows_in_flight))
+    return fail(CHM_E_INVALID, "unsupported tuning %d/%d", chk.panel_chunks,
+                chk.rows_in_flight);
   h->tune = d;
   if (h->gexec) {
     (void)hipGraphExecDestroy(h->gexec);

@@ -123,7 +123,7 @@ struct EdgeOut {
 template <int NC>
 struct RowRegs {
   double2 v[NC];
-  double eL, eR;  // lane 0: column c0-1; lane 63: column c0+128*NC
+  double e;  // lane 0: column c0-1; lane 63: column c0+128*NC (east edge)
 };
 
 template <int NC>
@@ -134,9 +134,7 @@ struct Panel {
   int w, h;    // interior of the array
   int c0;      // first interior column of the panel (1-based)
   int lane;
-  bool right_edge;  // the panel's east neighbour column is loaded separately
-  int col[NC];
-  bool ld[NC], st2[NC], st1[NC];
+  int ecol;    // column this lane loads into RowRegs::e (-1: none)
 
   __device__ inline void setup(const double* in_, double* out_, int64_t p,
                                int w_, int h_, int c0_) {
@@ -147,28 +145,23 @@ struct Panel {
     h = h_;
     c0 = c0_;
     lane = __lane_id();
-#pragma unroll
-    for (int k = 0; k < NC; k++) {
-      int c = c0 + 128 * k + 2 * lane;
-      col[k] = c;
-      ld[k] = c <= w + 1;
-      st2[k] = c + 1 <= w;
-      st1[k] = c == w;
-    }
-    right_edge = (c0 + 128 * NC) <= w + 1;
+    ecol = -1;
+    if (lane == 0) ecol = c0 - 1;
+    if (lane == 63 && c0 + 128 * NC <= w + 1) ecol = c0 + 128 * NC;
   }
+  __device__ inline int col(int k) const { return c0 + 128 * k + 2 * lane; }
 
   __device__ inline void load(RowRegs<NC>& r, int row) const {
     const double* rp = in + (int64_t)row * pitch + kColOff;
 #pragma unroll
     for (int k = 0; k < NC; k++) {
-      if (ld[k])
-        r.v[k] = *reinterpret_cast<const double2*>(rp + col[k]);
+      const int c = col(k);
+      if (c <= w + 1)
+        r.v[k] = *reinterpret_cast<const double2*>(rp + c);
       else
         r.v[k] = make_double2(0.0, 0.0);
     }
-    r.eL = (lane == 0) ? rp[c0 - 1] : 0.0;
-    r.eR = (lane == 63 && right_edge) ? rp[c0 + 128 * NC] : 0.0;
+    r.e = (ecol >= 0) ? rp[ecol] : 0.0;
   }
 
   template <bool RES, bool PACK>
@@ -181,38 +174,37 @@ struct Panel {
       const double a = cu.v[k].x, b = cu.v[k].y;
       double bl = from_left_lane(b);
       double ar = from_right_lane(a);
-      if (lane == 0) bl = (k == 0) ? cu.eL : readlane_d(cu.v[k - 1].y, 63);
+      if (lane == 0) bl = (k == 0) ? cu.e : readlane_d(cu.v[k - 1].y, 63);
       if (lane == 63)
-        ar = (k == NC - 1) ? cu.eR : readlane_d(cu.v[(k + 1) % NC].x, 0);
+        ar = (k == NC - 1) ? cu.e : readlane_d(cu.v[(k + 1) % NC].x, 0);
       // c + W + E + N + S, then * 0.2 (jacobi2d.cu:332-333)
       const double na = (a + bl + b + up.v[k].x + dn.v[k].x) * 0.2;
       const double nb = (b + a + ar + up.v[k].y + dn.v[k].y) * 0.2;
-      const int c = col[k];
-      if (st2[k]) {
+      const int c = col(k);
+      const bool two = c + 1 <= w, one = c == w;
+      if (two) {
         *reinterpret_cast<double2*>(orow + c) = make_double2(na, nb);
         if (RES) {
           acc_add(acc, res_term(na, a));
           acc_add(acc, res_term(nb, b));
         }
-      } else if (st1[k]) {
+      } else if (one) {
         orow[c] = na;
         if (RES) acc_add(acc, res_term(na, a));
       }
-      if (PACK) {
-        if (st2[k] || st1[k]) {
-          if (row == 1 && eo.top) {
-            eo.top[c - 1] = na;
-            if (st2[k]) eo.top[c] = nb;
-          }
-          if (row == h && eo.bottom) {
-            eo.bottom[c - 1] = na;
-            if (st2[k]) eo.bottom[c] = nb;
-          }
-          if (c == 1 && eo.left) eo.left[row - 1] = na;
-          if (eo.right) {
-            if (c == w) eo.right[row - 1] = na;
-            if (st2[k] && c + 1 == w) eo.right[row - 1] = nb;
-          }
+      if (PACK && (two || one)) {
+        if (row == 1 && eo.top) {
+          eo.top[c - 1] = na;
+          if (two) eo.top[c] = nb;
+        }
+        if (row == h && eo.bottom) {
+          eo.bottom[c - 1] = na;
+          if (two) eo.bottom[c] = nb;
+        }
+        if (c == 1 && eo.left) eo.left[row - 1] = na;
+        if (eo.right) {
+          if (c == w) eo.right[row - 1] = na;
+          if (two && c + 1 == w) eo.right[row - 1] = nb;
         }
       }
     }
@@ -221,54 +213,32 @@ struct Panel {
 
 // Sweep rows [r0, r1) (1-based, r1 <= h+1) of the panel starting at column
 // c0.  Reads rows r0-1 .. r1 of `in`, writes rows r0 .. r1-1 of `out`.
+// Registers hold a ring of U+2 rows: computing row r frees row r-1's slot,
+// which immediately receives the load of row r+U+1, so U rows stay in
+// flight.  The row loop is unrolled by the ring length so every slot index
+// is static.
 template <int NC, int U, bool RES, bool PACK>
 __device__ inline void sweep_panel(const double* __restrict__ in,
                                    double* __restrict__ out, int64_t pitch,
                                    int w, int h, int c0, int r0, int r1,
                                    u128acc& acc, const EdgeOut& eo) {
   if (r0 >= r1) return;
+  constexpr int K = U + 2;
   Panel<NC> p;
   p.setup(in, out, pitch, w, h, c0);
-  RowRegs<NC> up, cu, A[U], B[U];
-  p.load(up, r0 - 1);
-  p.load(cu, r0);
+  RowRegs<NC> R[K];
 #pragma unroll
-  for (int u = 0; u < U; u++)
-    if (r0 + 1 + u <= r1) p.load(A[u], r0 + 1 + u);
-
-  int r = r0;
-  for (;;) {
-    // prefetch the group after this one, then compute this one (in A)
+  for (int s = 0; s < K; s++)
+    if (r0 - 1 + s <= r1) p.load(R[s], r0 - 1 + s);
+  for (int r = r0; r < r1; r += K) {
 #pragma unroll
-    for (int u = 0; u < U; u++)
-      if (r + U + 1 + u <= r1) p.load(B[u], r + U + 1 + u);
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      if (r + u < r1) {
-        const RowRegs<NC>& c = (u == 0) ? cu : A[u - 1];
-        const RowRegs<NC>& n = (u == 0) ? up : ((u == 1) ? cu : A[u - 2]);
-        p.template compute<RES, PACK>(n, c, A[u], r + u, acc, eo);
+    for (int s = 0; s < K; s++) {
+      if (r + s < r1) {
+        p.template compute<RES, PACK>(R[s], R[(s + 1) % K], R[(s + 2) % K],
+                                      r + s, acc, eo);
+        if (r + s + U + 1 <= r1) p.load(R[s], r + s + U + 1);
       }
     }
-    up = (U == 1) ? cu : A[U - 2];
-    cu = A[U - 1];
-    r += U;
-    if (r >= r1) break;
-#pragma unroll
-    for (int u = 0; u < U; u++)
-      if (r + U + 1 + u <= r1) p.load(A[u], r + U + 1 + u);
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      if (r + u < r1) {
-        const RowRegs<NC>& c = (u == 0) ? cu : B[u - 1];
-        const RowRegs<NC>& n = (u == 0) ? up : ((u == 1) ? cu : B[u - 2]);
-        p.template compute<RES, PACK>(n, c, B[u], r + u, acc, eo);
-      }
-    }
-    up = (U == 1) ? cu : B[U - 2];
-    cu = B[U - 1];
-    r += U;
-    if (r >= r1) break;
   }
 }
 

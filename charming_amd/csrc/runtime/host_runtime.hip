@@ -295,6 +295,8 @@ int chm_init(int argc, char** argv, const chm_config* cfg_in) {
     fflush(stdout);
   }
   memset(&r.last, 0, sizeof(r.last));
+  r.last.n_pes = n_pes;
+  r.last.threads_per_pe = threads;
   r.inited = true;
   return CHM_OK;
 }

@@ -26,7 +26,11 @@
 #define CHARE_TYPE_CNT_MAX 1024  // chare.h:9 of the reference
 #define EM_CNT_MAX 128           // chare.h:10
 #define ASYNC_WAIT_MAX 16        // chare.h:11
-#define MISMATCH_MAX 128         // chare.h:12
+#define MISMATCH_MAX 128         // chare.h:12 (per proxy per PE there)
+// Deferred-message slots per PE here (all proxies together).
+#ifndef CHARMING_DEFERRED_MAX
+#define CHARMING_DEFERRED_MAX 1024
+#endif
 
 namespace charm {
 struct chare_proxy_base;
@@ -96,8 +100,8 @@ struct PeState {
   int n_deferred;      // entries in state 1 or 2
   int n_revived;       // entries in state 2
   int ended;           // charm::end() already called on this PE
-  int pad0;
-  DeferredEntry deferred[MISMATCH_MAX];
+  int hw;              // deferred entries in use lie below this index
+  DeferredEntry deferred[CHARMING_DEFERRED_MAX];
   Stats stats;
 };
 

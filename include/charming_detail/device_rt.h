@@ -51,7 +51,7 @@ struct Lds {
   int arg;
   int n;
   int pad;
-  uint32_t order[MISMATCH_MAX];
+  uint32_t order[CHARMING_DEFERRED_MAX];
 };
 static __shared__ Lds s_lds;
 
@@ -251,7 +251,7 @@ __device__ void compact() {
   Lds& L = lds();
   if (threadIdx.x == 0) {
     int n = 0;
-    for (int i = 0; i < MISMATCH_MAX; i++)
+    for (int i = 0; i < ps->hw; i++)
       if (ps->deferred[i].state != 0) {
         int k = n++;  // insertion sort by offset
         while (k > 0 && ps->deferred[L.order[k - 1]].offset >
@@ -291,11 +291,12 @@ __device__ void defer(const char* msg, uint64_t alloc, int chare_id,
   if (L.arg) compact();
   if (threadIdx.x == 0) {
     int e = -1;
-    for (int i = 0; i < MISMATCH_MAX; i++)
+    for (int i = 0; i < CHARMING_DEFERRED_MAX; i++)
       if (ps->deferred[i].state == 0) {
         e = i;
         break;
       }
+    if (e >= ps->hw) ps->hw = e + 1;
     if (e < 0 || ps->arena_used + (uint64_t)need > g_ctx.deferred_bytes) {
       raise_error(ERR_DEFERRED_FULL, (uint32_t)chare_idx);
       L.arg = -1;
@@ -326,7 +327,7 @@ __device__ void defer(const char* msg, uint64_t alloc, int chare_id,
 
 // Collective.  Decodes the envelope (scheduler.cu:23-76) and runs the entry
 // method through the refnum gate; false = must be deferred.
-__device__ bool dispatch(envelope* env) {
+__device__ __noinline__ bool dispatch(envelope* env) {
   if (env->type == msgtype::regular || env->type == msgtype::user) {
     regular_msg* m = (regular_msg*)((char*)env + sizeof(envelope));
     ProxyTable& t = pe_state()->table;
@@ -354,7 +355,7 @@ __device__ int poll_next() {
   for (uint32_t spins = 0;; spins++) {
     if (ps->n_revived > 0) {
       int best = -1;
-      for (int i = 0; i < MISMATCH_MAX; i++)
+      for (int i = 0; i < ps->hw; i++)
         if (ps->deferred[i].state == 2 &&
             (best < 0 || ps->deferred[i].seq < ps->deferred[best].seq))
           best = i;
@@ -385,7 +386,7 @@ __device__ int poll_next() {
   }
 }
 
-__device__ void process_inbox() {
+__device__ __noinline__ void process_inbox() {
   const Ctx& c = g_ctx;
   PeState* ps = pe_state();
   envelope* env = (envelope*)lds().ptr;
@@ -410,7 +411,7 @@ __device__ void process_inbox() {
   }
 }
 
-__device__ void process_revived() {
+__device__ __noinline__ void process_revived() {
   PeState* ps = pe_state();
   const int e = lds().arg;
   envelope* env = (envelope*)lds().ptr;
@@ -423,14 +424,17 @@ __device__ void process_revived() {
       ps->n_deferred--;
       ps->stats.delivered++;
       ps->stats.revived++;
-      if (ps->n_deferred == 0) ps->arena_used = 0;
+      if (ps->n_deferred == 0) {
+        ps->arena_used = 0;
+        ps->hw = 0;
+      }
     } else {
       d.state = 1;  // refnum moved on again: stay deferred
     }
   }
 }
 
-__device__ void pe_loop() {
+__device__ __noinline__ void pe_loop() {
   Lds& L = lds();
   for (;;) {
     if (threadIdx.x == 0) L.cmd = poll_next();
@@ -488,7 +492,8 @@ __device__ void pe_begin() {
     ps->n_deferred = 0;
     ps->n_revived = 0;
     ps->ended = 0;
-    for (int i = 0; i < MISMATCH_MAX; i++) ps->deferred[i].state = 0;
+    ps->hw = 0;
+    for (int i = 0; i < CHARMING_DEFERRED_MAX; i++) ps->deferred[i].state = 0;
     ps->stats = Stats{};
   }
   __syncthreads();
@@ -511,7 +516,7 @@ __device__ void send_reg_msg(int chare_id, int chare_idx, int ep_id, void* buf,
 __device__ void revive_mismatches(int chare_id, int chare_idx, int refnum) {
   rt::PeState* ps = rt::pe_state();
   if (ps->n_deferred == 0) return;
-  for (int i = 0; i < MISMATCH_MAX; i++) {
+  for (int i = 0; i < ps->hw; i++) {
     rt::DeferredEntry& d = ps->deferred[i];
     if (d.state == 1 && d.chare_id == chare_id && d.chare_idx == chare_idx &&
         d.refnum == refnum) {
@@ -571,15 +576,27 @@ __device__ double wall_time() {
 namespace charm {
 namespace rt {
 
-template <int kThreads>
-__global__ void __launch_bounds__(kThreads) scheduler_kernel(int unused) {
-  (void)unused;
+// The user's main and the loop run in (non-inlined) functions so the kernel
+// body keeps almost nothing live across calls: every function called through
+// an entry-method pointer is compiled for up to 1024-thread workgroups
+// (128 VGPRs), and the kernel must not add registers of its own on top.
+__device__ __noinline__ void run_user_main() {
+  charm::main(g_ctx.argc, g_ctx.argv, g_ctx.argvs, (int)blockIdx.x);
+}
+
+__device__ __noinline__ void pe_run() {
   pe_begin();
   grid_barrier();
-  charm::main(g_ctx.argc, g_ctx.argv, g_ctx.argvs, (int)blockIdx.x);
+  run_user_main();
   grid_barrier();
   pe_loop();
   grid_barrier();
+}
+
+template <int kThreads>
+__global__ void __launch_bounds__(kThreads) scheduler_kernel(int unused) {
+  (void)unused;
+  pe_run();
 }
 
 static int set_ctx_host(const void* ctx, size_t bytes) {
@@ -596,7 +613,7 @@ struct ProgramRegistrar {
     p.threads_per_pe = CHARMING_PE_THREADS;
     p.main_host = &charm::main_host;
     p.set_ctx = &set_ctx_host;
-    p.name = __FILE__;
+    p.name = __BASE_FILE__;  // the including program, not this header
     chm_register_program(&p);
   }
 };
