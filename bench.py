@@ -1,0 +1,182 @@
+"""Benchmark of the FP64 Jacobi2D hot path on one MI355X.
+
+Workload (the 16384^2 configuration): a 16384 x 16384 interior, seeded
+uniform [0,1) with boundary 1.0, decomposed into 16 x 16 chares of 1024^2
+with message-driven halo exchange, 1000 iterations, run in one persistent
+launch of the charming_amd scheduler.  One step = one 1000-iteration solve
+on the grid already resident in HBM (each step continues from the previous
+state; 1000 is even, so the buffer parity returns to the start).
+
+Prints ONE JSON line.  `value` is whole-job throughput in Gcell-upd/s
+(interior cells x iterations / wall seconds of the timed steps).  Beside it:
+the plain per-launch and hipGraph forms of the same sweep, the measured copy
+ceiling, the roofline of the dominant kernel (16 algorithmic bytes per
+cell-update over its HIP-event duration) and the CPU baseline (the oracle's
+threaded message-driven runner on the host cores, bounded sample).
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+W = H = 16384
+CHARES = (16, 16)
+ITERS = 1000
+SEED = 0xC4A40003
+PEAK_HBM_GBS = 8000.0
+BYTES_PER_CELL = 16.0  # read u once, write u' once (FP64)
+
+
+def _p(q, xs):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    k = min(len(xs) - 1, max(0, int(round(q / 100.0 * (len(xs) - 1)))))
+    return xs[k]
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the persistent scheduler, measured with
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and calibrated against a copy
+    kernel of known size (profiles/pmc_traffic.json), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        if d.get("workload") == f"{W}x{H}/{CHARES[0]}x{CHARES[1]}/{ITERS}":
+            return d.get("bytes_per_launch")
+    except (OSError, ValueError):
+        pass
+    return None
+
+
+def cpu_baseline(budget_s):
+    """The oracle's message-driven chare runner on host threads, over a
+    bounded sample of the same workload (same grid and chares, fewer
+    iterations)."""
+    from oracle import oracle as O
+    cores = min(16, len(os.sched_getaffinity(0)))
+    iters = 2
+    O.jacobi_chares_run(W, H, CHARES[0], CHARES[1], iters, cores, seed=SEED,
+                        want_grid=False)
+    t = O.last_run_seconds()
+    # scale the iteration count so the timed sample is about budget_s
+    iters = max(2, min(200, int(budget_s / max(t / 2, 1e-6))))
+    O.jacobi_chares_run(W, H, CHARES[0], CHARES[1], iters, cores, seed=SEED,
+                        want_grid=False)
+    t = O.last_run_seconds()
+    return {"value": W * H * iters / t / 1e9, "unit": "Gcell-upd/s",
+            "cores": cores, "kind": "port",
+            "sample": f"{W}x{H} interior, {CHARES[0]}x{CHARES[1]} chares, "
+                      f"{iters} of the {ITERS} iterations, message-driven "
+                      f"threads; iteration phase timed ({t:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-compare", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not "
+              "supported (this build runs on a single MI355X)", file=sys.stderr)
+        sys.exit(2)
+
+    import charming_amd as cam
+    L = cam.jacobi_lib()
+
+    j = cam.Jacobi2D(W, H, ITERS, form="persistent", seed=SEED, chares=CHARES)
+    for _ in range(args.warmup):
+        j.solve()
+    kern_ms, step_ms = [], []
+    msgs = deferred = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        r = j.solve()
+        step_ms.append((time.perf_counter() - s0) * 1e3)
+        kern_ms.append(r.device_ms)
+        msgs, deferred = r.messages, r.deferred
+    wall = time.perf_counter() - t0
+    j.close()
+
+    cells = float(W) * H * ITERS
+    value = cells * args.steps / wall / 1e9
+    kavg = statistics.mean(kern_ms)
+    achieved = BYTES_PER_CELL * cells / (kavg / 1e3) / 1e9
+
+    out = {
+        "metric": "Jacobi2D Gcell-upd/s & HBM GB/s; p50/p99 solve latency vs "
+                  "plain, 1 GPU",
+        "value": round(value, 3),
+        "unit": "Gcell-upd/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (splitmix64 uniform [0,1) interior, boundary 1.0)",
+        "config": {"workload": f"jacobi2d {W}x{H}, {CHARES[0]}x{CHARES[1]} "
+                               f"chares, {ITERS} iterations, persistent "
+                               "message-driven",
+                   "grid": [W, H], "chares": list(CHARES), "iters": ITERS,
+                   "form": "persistent", "parallelism": "1 GPU"},
+        "hbm_gbs": round(BYTES_PER_CELL * value, 1),
+        "latency_ms": {"p50": round(_p(50, step_ms), 3),
+                       "p99": round(_p(99, step_ms), 3)},
+        "messages_per_solve": msgs,
+        "deferred_per_solve": deferred,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": traffic_from_profiles(),
+                     "kernel": "charm::rt::scheduler_kernel<1024>",
+                     "kernel_ms": round(kavg, 3)},
+    }
+
+    if not args.no_compare:
+        import ctypes
+        ms = ctypes.c_double()
+        L.chm_hbm_copy_probe.argtypes = [ctypes.c_size_t, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_double)]
+        if L.chm_hbm_copy_probe(4 << 30, 5, ctypes.byref(ms)) == 0:
+            out["copy_ceiling_gbs"] = round(2 * (4 << 30) / ms.value / 1e6, 1)
+        comp = {}
+        for form in ("plain", "graph"):
+            p = cam.Jacobi2D(W, H, ITERS, form=form, seed=SEED)
+            p.solve()
+            p.reset()
+            lat = []
+            for _ in range(max(2, args.steps // 2)):
+                s0 = time.perf_counter()
+                r = p.solve()
+                lat.append((time.perf_counter() - s0) * 1e3)
+                p.reset()
+            p.close()
+            g = cells / (statistics.median(lat) / 1e3) / 1e9
+            comp[form] = {"gcell_per_s": round(g, 2),
+                          "hbm_gbs": round(BYTES_PER_CELL * g, 1),
+                          "latency_ms": {"p50": round(_p(50, lat), 3),
+                                         "p99": round(_p(99, lat), 3)},
+                          "launches_per_solve": ITERS}
+        out["compare"] = comp
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -34,6 +34,10 @@ LIBS = [
     ("libcharming_amd.so", ["runtime/host_runtime.hip"], []),
     ("libchm_jacobi2d.so", ["jacobi2d/plain.hip", "jacobi2d/persistent.hip"],
      ["charming_amd"]),
+    # the example chare programs as loadable libraries (run through chm_main)
+    ("libchm_hello.so", ["examples/hello.hip"], ["charming_amd"]),
+    ("libchm_refnum.so", ["examples/refnum.hip"], ["charming_amd"]),
+    ("libchm_pingpong.so", ["examples/pingpong.hip"], ["charming_amd"]),
 ]
 
 # chare-program executables: the program's TU + the library-provided main

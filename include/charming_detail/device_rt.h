@@ -67,8 +67,24 @@ __device__ inline uint32_t ld_relaxed(const uint32_t* p) {
 __device__ inline void st_relaxed(uint64_t* p, uint64_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Release store at agent scope.  Spelled out as fence + explicit wait +
+// relaxed store: the compiler's own release sequence (L2 writeback, then
+// wait, then store) loses its wait when the wave's counters are already
+// known to be drained, since the writeback itself is not tracked; the
+// store could then overtake the writeback of the data it publishes.
+__device__ inline void release_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 __device__ inline void st_release(uint64_t* p, uint64_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  release_fence();
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_release(uint32_t* p, uint32_t v) {
+  release_fence();
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ inline void fence_acquire() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -95,8 +111,7 @@ __device__ void raise_error(uint32_t code, uint32_t aux) {
     __hip_atomic_store(&ctl[CTL_ERROR_AUX], aux, __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
   }
-  __hip_atomic_store(&ctl[CTL_TERM], 1u, __ATOMIC_RELEASE,
-                     __HIP_MEMORY_SCOPE_AGENT);
+  st_release(&ctl[CTL_TERM], 1u);
 }
 
 __device__ inline bool stopping() {
@@ -457,12 +472,13 @@ __device__ void grid_barrier() {
     uint32_t* cnt = &g_ctx.ctl[CTL_BAR_COUNT];
     uint32_t* gen = &g_ctx.ctl[CTL_BAR_GEN];
     const uint32_t g = ld_relaxed(gen);
+    release_fence();
     const uint32_t arrived = __hip_atomic_fetch_add(
-        cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (arrived == (uint32_t)g_ctx.n_pes - 1) {
+      fence_acquire();
       __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(gen, g + 1, __ATOMIC_RELEASE,
-                         __HIP_MEMORY_SCOPE_AGENT);
+      st_release(gen, g + 1);
     } else {
       for (uint32_t spins = 0; ld_relaxed(gen) == g; spins++) {
         if ((spins & 255) == 255 && *g_ctx.host_abort) {
@@ -530,8 +546,7 @@ __device__ void revive_mismatches(int chare_id, int chare_idx, int refnum) {
 // every PE's scheduler loop; messages still in flight are dropped, as in
 // the reference (no quiescence detection).  Any thread may call it.
 __device__ void send_begin_term_msg() {
-  __hip_atomic_store(&rt::g_ctx.ctl[rt::CTL_TERM], 1u, __ATOMIC_RELEASE,
-                     __HIP_MEMORY_SCOPE_AGENT);
+  rt::st_release(&rt::g_ctx.ctl[rt::CTL_TERM], 1u);
 }
 
 __device__ void end() {

@@ -48,6 +48,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #define GOLDEN 0x9E3779B97F4A7C15ULL
 #define RES_SCALE 0x1.0p56
@@ -548,6 +549,12 @@ static void chare_init(runner_t* R, chare_t* c, uint64_t seed, int mode,
     }
 }
 
+static double g_last_run_seconds = 0.0;
+
+/* Wall time of the threaded (message-driven) phase of the last
+ * orc_jacobi_chares_run: iterations only, without init or gather. */
+double orc_last_run_seconds(void) { return g_last_run_seconds; }
+
 /* Run the message-driven decomposition: chare grid cr x cc over a W x H
  * interior, n_pes host threads, `iters` iterations.  out (if non-NULL) gets
  * the full (H+2)x(W+2) grid; res_* the exact residual of the last sweep.
@@ -581,9 +588,13 @@ long orc_jacobi_chares_run(int W, int H, int cr, int cc, int iters, int n_pes,
   for (int i = 0; i < R.n_chares; i++) send_boundaries(&R, &R.ch[i]);
   for (int i = 0; i < R.n_chares; i++)
     if (R.ch[i].nbr_count == 0) update(&R, &R.pes[R.loc[i]], &R.ch[i]);
+  struct timespec t0, t1;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
   pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * R.n_pes);
   for (int q = 0; q < R.n_pes; q++) pthread_create(&th[q], NULL, pe_main, &R.pes[q]);
   for (int q = 0; q < R.n_pes; q++) pthread_join(th[q], NULL);
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  g_last_run_seconds = (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
   long deferred = 0;
   for (int q = 0; q < R.n_pes; q++) {
     deferred += R.pes[q].deferred_count;

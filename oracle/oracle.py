@@ -62,6 +62,7 @@ def lib():
                                                C.c_double, C.c_void_p,
                                                C.POINTER(u64), C.POINTER(u64)]
         L.orc_jacobi_chares_hashes.restype = C.c_long
+        L.orc_last_run_seconds.restype = C.c_double
         _lib = L
     return _lib
 
@@ -160,6 +161,10 @@ def jacobi_chares_run(W, H, cr, cc, iters, n_pes, seed=0, mode=0, boundary=1.0,
                                     C.byref(lo), C.byref(hi))
     assert d >= 0, d
     return out, (hi.value << 64) | lo.value, d
+
+
+def last_run_seconds():
+    return lib().orc_last_run_seconds()
 
 
 def jacobi_chares_hashes(W, H, cr, cc, iters, n_pes, seed=0, mode=0, boundary=1.0):

@@ -15,7 +15,8 @@ def declared(header):
     return sorted(set(re.findall(r"\b(chm_[a-z0-9_]+)\s*\(", txt)))
 
 
-@pytest.mark.parametrize("header,lib", [("chm_jacobi2d.h", "libchm_jacobi2d.so")])
+@pytest.mark.parametrize("header,lib", [("chm_jacobi2d.h", "libchm_jacobi2d.so"),
+                                        ("charming_amd.h", "libcharming_amd.so")])
 def test_library_exports_declared_symbols(header, lib):
     path = os.path.join(ROOT, "charming_amd", "lib", lib)
     assert os.path.exists(path), f"{path} not built"
@@ -29,3 +30,9 @@ def test_library_exports_declared_symbols(header, lib):
 def test_jacobi_abi_version_without_gpu():
     import charming_amd
     assert charming_amd.jacobi_lib().chm_jacobi2d_abi_version() == 1
+
+
+@pytest.mark.parametrize("prog", ["hello", "refnum", "pingpong"])
+def test_example_programs_built(prog):
+    assert os.path.exists(os.path.join(ROOT, "charming_amd", "lib", f"libchm_{prog}.so"))
+    assert os.path.exists(os.path.join(ROOT, "charming_amd", "bin", prog))

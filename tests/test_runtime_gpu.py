@@ -1,34 +1,58 @@
-"""The persistent message-driven runtime on the GPU: the reference's own
-verification method (run the example programs, read what they print),
-with exact checks on delivery order and count, refnum deferral/revival,
-payload bytes and termination."""
+"""The persistent message-driven runtime on the GPU, checked the way the
+reference checks its examples (run the program, read what it prints), with
+exact checks on delivery order and count, refnum deferral/revival, payload
+bytes and termination.  Programs run in this process through the C ABI
+(chm_select_program + chm_main): a cooperative launch cannot be made from a
+second process while this one holds the device's cooperative resources."""
+import ctypes
 import os
 import re
-import subprocess
 
 import pytest
 
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-BIN = os.path.join(ROOT, "charming_amd", "bin")
+LIB = os.path.join(ROOT, "charming_amd", "lib")
+_loaded = {}
 
 
-def run(prog, *args, n_pes=None, timeout=60):
-    env = dict(os.environ, CHM_TIMEOUT_S="40", CHM_STATS="1")
+def runtime():
+    if "rt" not in _loaded:
+        rt = ctypes.CDLL(os.path.join(LIB, "libcharming_amd.so"), mode=ctypes.RTLD_GLOBAL)
+        rt.chm_main.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]
+        rt.chm_select_program.argtypes = [ctypes.c_char_p]
+        rt.chm_last_error.restype = ctypes.c_char_p
+        _loaded["rt"] = rt
+    return _loaded["rt"]
+
+
+def run(capfd, prog, *args, n_pes=None):
+    rt = runtime()
+    if prog not in _loaded:
+        _loaded[prog] = ctypes.CDLL(os.path.join(LIB, f"libchm_{prog}.so"))
+    assert rt.chm_select_program(f"{prog}.hip".encode()) == 0
+    os.environ["CHM_TIMEOUT_S"] = "40"
+    os.environ["CHM_STATS"] = "1"
     if n_pes:
-        env["CHM_N_PES"] = str(n_pes)
+        os.environ["CHM_N_PES"] = str(n_pes)
+    else:
+        os.environ.pop("CHM_N_PES", None)
+    argv = [prog.encode()] + [str(a).encode() for a in args]
+    arr = (ctypes.c_char_p * len(argv))(*argv)
     print(f"running {prog} {args} n_pes={n_pes}", flush=True)
-    p = subprocess.run([os.path.join(BIN, prog)] + [str(a) for a in args],
-                       capture_output=True, text=True, timeout=timeout, env=env)
-    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
-    stats = dict(re.findall(r"(\w+)=([\d.]+)", p.stdout.split("[STATS]")[-1]))
-    return p.stdout, stats
+    capfd.readouterr()
+    rc = rt.chm_main(len(argv), arr)
+    out, err = capfd.readouterr()
+    os.environ.pop("CHM_N_PES", None)
+    assert rc == 0, out[-2000:] + err[-2000:]
+    stats = dict(re.findall(r"(\w+)=([\d.]+)", out.split("[STATS]")[-1]))
+    return out, stats
 
 
 @pytest.mark.parametrize("n_pes", [None, 4, 1])
-def test_hello_chain(n_pes):
-    out, st = run("hello", n_pes=n_pes)
+def test_hello_chain(capfd, n_pes):
+    out, st = run(capfd, "hello", n_pes=n_pes)
     lines = re.findall(r"Hello I'm (\d+) of (\d+)! Received (\d+)", out)
     P = int(st["pes"])
     n = 2 * P
@@ -40,16 +64,16 @@ def test_hello_chain(n_pes):
 
 
 @pytest.mark.parametrize("n_pes", [None, 2])
-def test_refnum_deferral_and_revival(n_pes):
-    out, st = run("refnum", n_pes=n_pes)
+def test_refnum_deferral_and_revival(capfd, n_pes):
+    out, st = run(capfd, "refnum", n_pes=n_pes)
     got = re.findall(r"Received message with refnum (\d+)", out)
     assert got == ["1", "0"]
     assert int(st["deferred"]) == 1 and int(st["revived"]) == 1
     assert int(st["error"]) == 0
 
 
-def test_pingpong_exact_payloads():
-    out, st = run("pingpong", 8, 1 << 20, 20, 2, 1)
+def test_pingpong_exact_payloads(capfd):
+    out, st = run(capfd, "pingpong", 8, 1 << 20, 20, 2, 1)
     sizes = re.findall(r"Size (\d+) took ([\d.]+) us", out)
     assert [int(s) for s, _ in sizes] == [8 << k for k in range(18)]
     m = re.search(r"round trips (\d+), payload errors (\d+)", out)
