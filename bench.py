@@ -85,6 +85,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="only run the streaming copy of a known size (1 GiB "
+                         "read + 1 GiB written per launch) used to scale "
+                         "traffic counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not "
@@ -93,6 +97,15 @@ def main():
 
     import charming_amd as cam
     L = cam.jacobi_lib()
+    if args.calibrate:
+        import ctypes
+        ms = ctypes.c_double()
+        L.chm_hbm_copy_probe.argtypes = [ctypes.c_size_t, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_double)]
+        rc = L.chm_hbm_copy_probe(1 << 30, 3, ctypes.byref(ms))
+        print(json.dumps({"calibration_copy_bytes": 1 << 30, "launches": 4,
+                          "ms_per_copy": ms.value, "rc": rc}), flush=True)
+        return
 
     j = cam.Jacobi2D(W, H, ITERS, form="persistent", seed=SEED, chares=CHARES)
     for _ in range(args.warmup):

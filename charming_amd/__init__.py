@@ -46,7 +46,8 @@ class Result(C.Structure):
     _fields_ = [("device_ms", C.c_double), ("residual_lo", C.c_uint64),
                 ("residual_hi", C.c_uint64), ("hash", C.c_uint64),
                 ("iters_done", C.c_int), ("launches", C.c_int),
-                ("messages", C.c_longlong), ("deferred", C.c_longlong)]
+                ("messages", C.c_longlong), ("deferred", C.c_longlong),
+                ("sched_overhead_ms", C.c_double)]
 
     @property
     def residual(self):
@@ -55,7 +56,8 @@ class Result(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("panel_chunks", C.c_int), ("rows_in_flight", C.c_int),
-                ("band_rows", C.c_int), ("wg_waves", C.c_int)]
+                ("band_rows", C.c_int), ("wg_waves", C.c_int),
+                ("overlap", C.c_int)]
 
 
 _jac = None
@@ -74,6 +76,7 @@ def jacobi_lib():
         L.chm_jacobi2d_hash_region.argtypes = [vp, C.c_int, C.c_int, C.c_int,
                                                C.c_int, C.POINTER(C.c_uint64)]
         L.chm_jacobi2d_destroy.argtypes = [vp]
+        L.chm_jacobi2d_phase_stats.argtypes = [vp, C.c_void_p]
         L.chm_jacobi2d_last_error.restype = C.c_char_p
         L.chm_jacobi2d_abi_version.restype = C.c_int
         _jac = L
@@ -107,8 +110,8 @@ class Jacobi2D:
         self.h = h
 
     def set_tuning(self, panel_chunks=0, rows_in_flight=0, band_rows=0,
-                   wg_waves=0):
-        t = Tuning(panel_chunks, rows_in_flight, band_rows, wg_waves)
+                   wg_waves=0, overlap=0):
+        t = Tuning(panel_chunks, rows_in_flight, band_rows, wg_waves, overlap)
         _check(jacobi_lib().chm_jacobi2d_set_tuning(self.h, C.byref(t)))
 
     def reset(self):
@@ -130,6 +133,15 @@ class Jacobi2D:
         _check(jacobi_lib().chm_jacobi2d_hash_region(self.h, r0, r1, c0, c1,
                                                      C.byref(v)))
         return v.value
+
+    def phase_stats(self):
+        """Per-chare ms spent in sweep/send/wait in the last persistent
+        solve: {'mean': [...], 'max': [...]}"""
+        out = np.zeros(8, dtype=np.float64)
+        _check(jacobi_lib().chm_jacobi2d_phase_stats(self.h, out.ctypes.data))
+        keys = ("sweep", "send", "wait", "recv")
+        return {"mean_ms": dict(zip(keys, out[:4].round(3).tolist())),
+                "max_ms": dict(zip(keys, out[4:].round(3).tolist()))}
 
     def close(self):
         if getattr(self, "h", None):

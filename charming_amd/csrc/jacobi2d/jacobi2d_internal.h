@@ -52,6 +52,7 @@ int persistent_solve(Handle* h, chm_jacobi_result* res);
 int persistent_read_grid(Handle* h, double* out);
 int persistent_hash_region(Handle* h, int r0, int r1, int c0, int c1,
                            uint64_t* hash);
+int persistent_phase_stats(Handle* h, double* out);
 void persistent_destroy(Handle* h);
 
 }  // namespace jac

@@ -45,8 +45,11 @@ struct JacobiDesc {
   double* staging;        // chare c: 4 edges at staging + (4c + d) * stage_elems
   int64_t stage_elems;
   uint64_t* res_acc;      // exact residual of the last sweep {lo, hi}
+  uint64_t* phase;        // per chare: wall-clock ticks in sweep, send, wait
   int W, H, cr, cc, iters;
   int tune_nc, tune_u;
+  int overlap;            // sweep edges first, send, then the interior
+  uint64_t* recv_ticks;   // per chare: ticks inside recv_ghosts (unpack)
 };
 
 __device__ JacobiDesc g_jdesc;
@@ -78,6 +81,9 @@ struct Block : charm::chare {
   int n_iters, iter;
   double* grids[2];      // u_k lives in grids[k & 1]
   double* boundaries[N_NEIGHBORS];  // [dir slot | edge] staging
+  // per-chare phase accounting (jacobi2d.h:31-42 keeps send/recv/update
+  // times the same way): device wall-clock ticks
+  uint64_t t_sweep, t_send, t_wait, t_mark;
 
   __device__ Block() {}
   __device__ void init(void* arg);
@@ -118,57 +124,108 @@ __device__ void charm::main(int argc, char** argv, size_t* argvs, int pe) {
   }
 }
 
-// The sweep of one chare by its PE's workgroup: waves take (panel, band)
-// tiles of the block.  RES accumulates the exact residual of this sweep.
-template <int NC, int U>
-__device__ void chare_sweep(const double* in, double* out, int64_t P, int bw,
-                            int bh, bool res, const EdgeOut& eo) {
-  const int wave = threadIdx.x / 64, W = blockDim.x / 64;
+// The sweep of one chare by its PE's workgroup, in two phases so the halo
+// sends overlap the bulk of the work: phase 0 sweeps the edge panels (first
+// and last 128*NC columns, full height) and thin top/bottom bands of the
+// middle panels, which produce every outgoing edge; phase 1 sweeps the rest.
+// Waves take tiles round-robin.  RES accumulates the exact residual.
+template <int NC, int U, bool RES>
+__device__ __noinline__ void chare_sweep(const double* in, double* out,
+                                         int64_t P, int bw, int bh,
+                                         EdgeOut eo, int phase) {
+  const int wave = threadIdx.x / 64, WV = blockDim.x / 64;
   const int pw = 128 * NC;
-  const int n_panels = (bw + pw - 1) / pw;
-  int n_bands = W / n_panels;
-  if (n_bands < 1) n_bands = 1;
-  if (n_bands > bh) n_bands = bh;
-  const int band_rows = (bh + n_bands - 1) / n_bands;
-  const int tiles = n_panels * n_bands;
+  const int S = (bw + pw - 1) / pw;  // panels
+  const int e = bh > 16 ? 4 : bh;     // rows of the thin bands
+  const int n_mid = S > 2 ? S - 2 : 0;
   u128acc acc = {0, 0};
-  for (int t = wave; t < tiles; t += W) {
-    const int panel = t % n_panels, band = t / n_panels;
-    const int r0 = 1 + band * band_rows;
-    const int r1 = min(r0 + band_rows, bh + 1);
-    const int c0 = 1 + panel * pw;
-    if (res)
-      sweep_panel<NC, U, true, true>(in, out, P, bw, bh, c0, r0, r1, acc, eo);
-    else
-      sweep_panel<NC, U, false, true>(in, out, P, bw, bh, c0, r0, r1, acc, eo);
+  if (phase == 2) {  // the whole block at once: panels x bands
+    int nb = WV / S;
+    if (nb < 1) nb = 1;
+    if (nb > bh) nb = bh;
+    const int br = (bh + nb - 1) / nb;
+    for (int t = wave; t < S * nb; t += WV) {
+      const int panel = t % S, b = t / S;
+      const int r0 = 1 + b * br, r1 = min(r0 + br, bh + 1);
+      sweep_panel<NC, U, RES, true>(in, out, P, bw, bh, 1 + panel * pw, r0,
+                                    r1, acc, eo);
+    }
+  } else if (phase == 0) {
+    const int n_edge = S == 1 ? 1 : 2;
+    int nb = WV / n_edge;
+    if (nb < 1) nb = 1;
+    if (nb > bh) nb = bh;
+    const int br = (bh + nb - 1) / nb;
+    const int tiles_edge = n_edge * nb;
+    for (int t = wave; t < tiles_edge + 2 * n_mid; t += WV) {
+      int panel, r0, r1;
+      if (t < tiles_edge) {
+        panel = (t / nb == 0) ? 0 : S - 1;
+        r0 = 1 + (t % nb) * br;
+        r1 = min(r0 + br, bh + 1);
+      } else {
+        const int q = t - tiles_edge;
+        panel = 1 + q / 2;
+        if (q % 2 == 0) {
+          r0 = 1;
+          r1 = min(1 + e, bh + 1);
+        } else {
+          r0 = max(1 + e, bh + 1 - e);
+          r1 = bh + 1;
+        }
+      }
+      sweep_panel<NC, U, RES, true>(in, out, P, bw, bh, 1 + panel * pw, r0,
+                                    r1, acc, eo);
+    }
+  } else if (n_mid > 0) {
+    const int rlo = 1 + e, rhi = max(1 + e, bh + 1 - e);
+    if (rhi > rlo) {
+      int nb = (3 * WV + n_mid - 1) / n_mid;  // about three tiles per wave
+      if (nb > rhi - rlo) nb = rhi - rlo;
+      const int br = (rhi - rlo + nb - 1) / nb;
+      for (int t = wave; t < n_mid * nb; t += WV) {
+        // band-major: neighbouring panels are swept at the same time
+        const int panel = 1 + t % n_mid, b = t / n_mid;
+        const int r0 = rlo + b * br, r1 = min(r0 + br, rhi);
+        sweep_panel<NC, U, RES, true>(in, out, P, bw, bh, 1 + panel * pw, r0,
+                                      r1, acc, eo);
+      }
+    }
   }
-  if (res) {
+  if (RES) {
     acc = wave_reduce_u128(acc);
     if (__lane_id() == 0) atomic_add_u128(g_jdesc.res_acc, acc);
   }
 }
 
 __device__ void sweep_dispatch(const double* in, double* out, int64_t P,
-                               int bw, int bh, bool res, const EdgeOut& eo) {
+                               int bw, int bh, bool res, const EdgeOut& eo,
+                               int phase) {
   int nc = g_jdesc.tune_nc, u = g_jdesc.tune_u;
   // Entry methods are reached through a function pointer, so their code is
   // limited to 128 VGPRs whatever the launch bounds: only tiles whose row
   // ring fits (NC * U <= 4) are instantiated; with 16 waves per PE that is
   // 16 x 4 KiB of reads in flight per CU.
   if (nc == 0) {
-    nc = bw > 256 ? 2 : 1;
-    u = 4 / nc;
+    nc = 1;
+    u = 2;
   }
   if (u == 0) u = (4 / nc) > 0 ? 4 / nc : 1;
-#define CHM_CASE(NC_, U_)                                   \
-  if (nc == NC_ && u == U_) {                               \
-    chare_sweep<NC_, U_>(in, out, P, bw, bh, res, eo);      \
-    return;                                                 \
+#define CHM_CASE(NC_, U_)                                               \
+  if (nc == NC_ && u == U_) {                                           \
+    if (res)                                                            \
+      chare_sweep<NC_, U_, true>(in, out, P, bw, bh, eo, phase);        \
+    else                                                                \
+      chare_sweep<NC_, U_, false>(in, out, P, bw, bh, eo, phase);       \
+    return;                                                             \
   }
-  CHM_CASE(1, 4) CHM_CASE(2, 2) CHM_CASE(4, 1) CHM_CASE(1, 2) CHM_CASE(2, 1)
-  CHM_CASE(1, 1)
+  CHM_CASE(1, 2) CHM_CASE(1, 4) CHM_CASE(2, 2) CHM_CASE(1, 1) CHM_CASE(2, 1)
+  CHM_CASE(4, 1)
 #undef CHM_CASE
-  chare_sweep<1, 4>(in, out, P, bw, bh, res, eo);
+  if (res)
+    chare_sweep<1, 2, true>(in, out, P, bw, bh, eo, phase);
+  else
+    chare_sweep<1, 2, false>(in, out, P, bw, bh, eo, phase);
 }
 
 __device__ void Block::init(void* arg) {
@@ -194,6 +251,7 @@ __device__ void Block::init(void* arg) {
       neighbor_count += (neighbor_index[k] >= 0);
     recv_count = 0;
     end_count = 0;
+    t_sweep = t_send = t_wait = 0;
     grids[0] = d.arena + (int64_t)(2 * index) * d.chare_elems;
     grids[1] = grids[0] + d.chare_elems;
     for (int k = 0; k < N_NEIGHBORS; k++) {
@@ -219,6 +277,7 @@ __device__ void Block::init(void* arg) {
   }
   __syncthreads();
   send_boundaries();
+  if (threadIdx.x == 0) t_mark = wall_clock64();
   if (neighbor_count == 0) update();
 }
 
@@ -235,6 +294,7 @@ __device__ void Block::send_boundaries() {
 
 __device__ void Block::recv_ghosts(void* arg) {
   // (jacobi2d.cu:196-229) unpack into the halo of u_iter
+  const uint64_t tr0 = wall_clock64();
   const int dir = *(const int*)arg;
   const double* ghost = (const double*)arg + 1;
   double* t = grids[iter & 1];
@@ -266,6 +326,8 @@ __device__ void Block::recv_ghosts(void* arg) {
   if (threadIdx.x == 0) {
     go = (++recv_count == neighbor_count);
     if (go) recv_count = 0;
+    if (g_jdesc.recv_ticks)
+      g_jdesc.recv_ticks[charm::chare::i] += wall_clock64() - tr0;
   }
   __syncthreads();
   if (go) update();
@@ -275,6 +337,11 @@ __device__ void Block::update() {
   const int bw = block_width, bh = block_height;
   const int64_t P = pitch;
   for (;;) {
+    uint64_t t0 = 0;
+    if (threadIdx.x == 0) {
+      t0 = wall_clock64();
+      t_wait += t0 - t_mark;
+    }
     const bool last = (iter == n_iters - 1);
     double* t = grids[iter & 1];
     double* nt = grids[(iter + 1) & 1];
@@ -283,20 +350,50 @@ __device__ void Block::update() {
     eo.bottom = (neighbor_index[BOTTOM] >= 0) ? boundaries[BOTTOM] + 1 : nullptr;
     eo.left = (neighbor_index[LEFT] >= 0) ? boundaries[LEFT] + 1 : nullptr;
     eo.right = (neighbor_index[RIGHT] >= 0) ? boundaries[RIGHT] + 1 : nullptr;
-    sweep_dispatch(t, nt, P, bw, bh, last, eo);
+    // overlap: phase 0 = everything that feeds the outgoing halos, sent
+    // before phase 1 sweeps the interior; otherwise one whole-block sweep
+    const bool ov = g_jdesc.overlap != 0;
+    if (ov)
+      sweep_dispatch(t, nt, P, bw, bh, last, eo, 0);
+    else
+      sweep_dispatch(t, nt, P, bw, bh, last, eo, 2);
     __syncthreads();
     __shared__ int done;
+    uint64_t t1 = 0, t2 = 0;
     if (threadIdx.x == 0) {
+      t1 = wall_clock64();
+      t_sweep += t1 - t0;
       ++iter;
       done = (iter == n_iters);
       if (!done) block_proxy->set_refnum(charm::chare::i, iter);
     }
     __syncthreads();
+    // the halos of u_{iter} leave before the interior is swept
+    if (!done) {
+      send_boundaries();
+      if (ov) charm::flush_sends();
+    }
+    if (threadIdx.x == 0) {
+      t2 = wall_clock64();
+      t_send += t2 - t1;
+    }
+    if (ov) sweep_dispatch(t, nt, P, bw, bh, last, eo, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      t_mark = wall_clock64();
+      t_sweep += t_mark - t2;
+    }
     if (done) {
+      if (threadIdx.x == 0 && g_jdesc.phase) {
+        uint64_t* ph = g_jdesc.phase + 4 * (int64_t)charm::chare::i;
+        ph[0] = t_sweep;
+        ph[1] = t_send;
+        ph[2] = t_wait;
+        ph[3] = (uint64_t)n_iters;
+      }
       block_proxy->invoke(0, 2);
       return;
     }
-    send_boundaries();
     if (neighbor_count > 0) return;
   }
 }
@@ -449,7 +546,11 @@ int persistent_create(Handle* h) {
       (size_t)ps->n_chares * 4 * (size_t)d.stage_elems * sizeof(double);
   if (hipMalloc(&d.arena, arena_bytes) != hipSuccess ||
       hipMalloc(&d.staging, stage_bytes) != hipSuccess ||
-      hipMalloc(&d.res_acc, 4 * sizeof(uint64_t)) != hipSuccess)
+      hipMalloc(&d.res_acc, 4 * sizeof(uint64_t)) != hipSuccess ||
+      hipMalloc(&d.phase, (size_t)ps->n_chares * 4 * sizeof(uint64_t)) !=
+          hipSuccess ||
+      hipMalloc(&d.recv_ticks, (size_t)ps->n_chares * sizeof(uint64_t)) !=
+          hipSuccess)
     return fail(CHM_E_NOMEM, "persistent form allocation (%zu + %zu bytes)",
                 arena_bytes, stage_bytes);
   return persistent_reset(h);
@@ -474,8 +575,10 @@ int persistent_solve(Handle* h, chm_jacobi_result* res) {
   // zero = choose per chare width on the device
   d.tune_nc = h->tune.panel_chunks;
   d.tune_u = h->tune.rows_in_flight;
+  d.overlap = h->tune.overlap;
   CHM_TRY(upload_desc(d));
   CHM_TRY_HIP(hipMemset(d.res_acc, 0, 4 * sizeof(uint64_t)));
+  CHM_TRY_HIP(hipMemset(d.recv_ticks, 0, (size_t)ps->n_chares * 8));
   CHM_TRY_HIP(hipDeviceSynchronize());
   if (d.iters == 0) {
     h->cur = 0;
@@ -493,6 +596,7 @@ int persistent_solve(Handle* h, chm_jacobi_result* res) {
     res->launches = d.iters ? 1 : 0;
     res->messages = (long long)st.delivered;
     res->deferred = (long long)st.deferred;
+    res->sched_overhead_ms = st.overhead_ms_per_pe;
     uint64_t acc[2];
     CHM_TRY_HIP(hipMemcpy(acc, d.res_acc, sizeof(acc), hipMemcpyDeviceToHost));
     res->residual_lo = acc[0];
@@ -535,12 +639,39 @@ int persistent_read_grid(Handle* h, double* out) {
   return CHM_OK;
 }
 
+int persistent_phase_stats(Handle* h, double* out) {
+  PersistentState* ps = h->ps;
+  std::vector<uint64_t> ph((size_t)ps->n_chares * 4);
+  CHM_TRY_HIP(hipMemcpy(ph.data(), ps->desc.phase, ph.size() * 8,
+                        hipMemcpyDeviceToHost));
+  int khz = 100000;
+  (void)hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0);
+  const double ms_per_tick = 1.0 / (double)khz;
+  std::vector<uint64_t> rt((size_t)ps->n_chares);
+  CHM_TRY_HIP(hipMemcpy(rt.data(), ps->desc.recv_ticks, rt.size() * 8,
+                        hipMemcpyDeviceToHost));
+  for (int c = 0; c < ps->n_chares; c++) ph[4 * c + 3] = rt[c];
+  for (int k = 0; k < 4; k++) {
+    double sum = 0, mx = 0;
+    for (int c = 0; c < ps->n_chares; c++) {
+      const double v = (double)ph[4 * c + k] * ms_per_tick;
+      sum += v;
+      mx = std::max(mx, v);
+    }
+    out[k] = sum / ps->n_chares;
+    out[4 + k] = mx;
+  }
+  return CHM_OK;
+}
+
 void persistent_destroy(Handle* h) {
   PersistentState* ps = h->ps;
   if (!ps) return;
   if (ps->desc.arena) (void)hipFree(ps->desc.arena);
   if (ps->desc.staging) (void)hipFree(ps->desc.staging);
   if (ps->desc.res_acc) (void)hipFree(ps->desc.res_acc);
+  if (ps->desc.phase) (void)hipFree(ps->desc.phase);
+  if (ps->desc.recv_ticks) (void)hipFree(ps->desc.recv_ticks);
   if (ps->own_runtime && --g_runtime_users == 0) chm_finalize();
   delete ps;
   h->ps = nullptr;

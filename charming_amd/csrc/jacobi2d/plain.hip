@@ -410,6 +410,14 @@ int chm_jacobi2d_hash_region(void* handle, int r0, int r1, int c0, int c1,
   return CHM_OK;
 }
 
+int chm_jacobi2d_phase_stats(void* handle, double* out8) {
+  Handle* h = (Handle*)handle;
+  if (!h || !out8) return fail(CHM_E_INVALID, "null argument");
+  if (h->form != CHM_FORM_PERSISTENT || !h->ps)
+    return fail(CHM_E_UNSUPPORTED, "phase times exist for the persistent form");
+  return persistent_phase_stats(h, out8);
+}
+
 int chm_hbm_copy_probe(size_t bytes, int reps, double* ms_per_copy) {
   if (bytes < 16 || reps < 1 || !ms_per_copy)
     return fail(CHM_E_INVALID, "bad copy probe arguments");

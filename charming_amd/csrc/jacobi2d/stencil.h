@@ -120,6 +120,23 @@ struct EdgeOut {
   double* right;   // column w, h values
 };
 
+// Pointers into global memory (address space 1): loads and stores through
+// them are global_* instructions even when the pointer came from a generic
+// one (chare state), not flat_* ones that must also assume LDS.
+typedef __attribute__((address_space(1))) const double gcdouble;
+typedef __attribute__((address_space(1))) double gdouble;
+typedef double v2d __attribute__((ext_vector_type(2)));
+__device__ inline double2 gload2(const gcdouble* p) {
+  const v2d v = *(const __attribute__((address_space(1))) v2d*)p;
+  return make_double2(v.x, v.y);
+}
+__device__ inline void gstore2(gdouble* p, double a, double b) {
+  v2d v;
+  v.x = a;
+  v.y = b;
+  *(__attribute__((address_space(1))) v2d*)p = v;
+}
+
 template <int NC>
 struct RowRegs {
   double2 v[NC];
@@ -128,8 +145,8 @@ struct RowRegs {
 
 template <int NC>
 struct Panel {
-  const double* in;
-  double* out;
+  const gcdouble* in;
+  gdouble* out;
   int64_t pitch;
   int w, h;    // interior of the array
   int c0;      // first interior column of the panel (1-based)
@@ -138,8 +155,8 @@ struct Panel {
 
   __device__ inline void setup(const double* in_, double* out_, int64_t p,
                                int w_, int h_, int c0_) {
-    in = in_;
-    out = out_;
+    in = (const gcdouble*)in_;
+    out = (gdouble*)out_;
     pitch = p;
     w = w_;
     h = h_;
@@ -152,12 +169,12 @@ struct Panel {
   __device__ inline int col(int k) const { return c0 + 128 * k + 2 * lane; }
 
   __device__ inline void load(RowRegs<NC>& r, int row) const {
-    const double* rp = in + (int64_t)row * pitch + kColOff;
+    const gcdouble* rp = in + (int64_t)row * pitch + kColOff;
 #pragma unroll
     for (int k = 0; k < NC; k++) {
       const int c = col(k);
       if (c <= w + 1)
-        r.v[k] = *reinterpret_cast<const double2*>(rp + c);
+        r.v[k] = gload2(rp + c);
       else
         r.v[k] = make_double2(0.0, 0.0);
     }
@@ -168,7 +185,7 @@ struct Panel {
   __device__ inline void compute(const RowRegs<NC>& up, const RowRegs<NC>& cu,
                                  const RowRegs<NC>& dn, int row,
                                  u128acc& acc, const EdgeOut& eo) const {
-    double* orow = out + (int64_t)row * pitch + kColOff;
+    gdouble* orow = out + (int64_t)row * pitch + kColOff;
 #pragma unroll
     for (int k = 0; k < NC; k++) {
       const double a = cu.v[k].x, b = cu.v[k].y;
@@ -183,7 +200,7 @@ struct Panel {
       const int c = col(k);
       const bool two = c + 1 <= w, one = c == w;
       if (two) {
-        *reinterpret_cast<double2*>(orow + c) = make_double2(na, nb);
+        gstore2(orow + c, na, nb);
         if (RES) {
           acc_add(acc, res_term(na, a));
           acc_add(acc, res_term(nb, b));

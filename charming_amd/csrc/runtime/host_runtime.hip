@@ -380,13 +380,16 @@ int chm_get_stats(chm_stats* out) {
                           hipMemcpyDeviceToHost));
   chm_stats o = r.last;
   o.delivered = o.sent = o.deferred = o.revived = o.send_spins = 0;
+  double ticks = 0;
   for (const Stats& x : s) {
+    ticks += (double)x.overhead_ticks;
     o.delivered += x.delivered;
     o.sent += x.sent;
     o.deferred += x.deferred;
     o.revived += x.revived;
     o.send_spins += x.send_spins;
   }
+  o.overhead_ms_per_pe = ticks / r.h.n_pes / r.h.wall_hz * 1e3;
   *out = o;
   return CHM_OK;
 }
@@ -413,9 +416,11 @@ int chm_main(int argc, char** argv) {
     if (st && atoi(st) && chm_get_stats(&s) == CHM_OK) {
       fflush(stdout);
       printf("[STATS] pes=%d threads=%d delivered=%llu sent=%llu deferred=%llu "
-             "revived=%llu send_spins=%llu kernel_ms=%.3f error=%u\n",
+             "revived=%llu send_spins=%llu kernel_ms=%.3f overhead_ms=%.3f "
+             "error=%u\n",
              s.n_pes, s.threads_per_pe, s.delivered, s.sent, s.deferred,
-             s.revived, s.send_spins, s.kernel_ms, s.error_code);
+             s.revived, s.send_spins, s.kernel_ms, s.overhead_ms_per_pe,
+             s.error_code);
     }
     chm_finalize();
   } else {

@@ -38,6 +38,13 @@ __device__ int device_atoi(const char* str, int strlen);
 // Seconds on the device wall clock (constant rate).
 __device__ double wall_time();
 
+// Messages sent by an entry method are published together when it returns
+// (one cache writeback for all of them).  flush_sends() publishes the ones
+// sent so far right away, for an entry method that keeps computing after
+// sending.  Workgroup-collective.  (No counterpart in the reference, whose
+// sends are published one by one.)
+__device__ void flush_sends();
+
 }  // namespace charm
 
 #ifndef CHARMING_NO_RUNTIME

@@ -44,6 +44,8 @@ typedef struct chm_stats {
   unsigned long long revived;    /* deferred messages re-dispatched */
   unsigned long long send_spins; /* polls spent waiting for ring space */
   double kernel_ms;              /* device time of the last chm_run */
+  double overhead_ms_per_pe;     /* mean time per PE in the scheduler around
+                                    entry methods (decode, fences, release) */
   int n_pes;
   int threads_per_pe;
   unsigned int error_code;       /* device error word of the last run (0 = none) */

@@ -88,7 +88,9 @@ struct Stats {
   unsigned long long revived;
   unsigned long long send_spins;  // polls while a ring was full
   unsigned long long idle_polls;
-  unsigned long long pad[2];
+  unsigned long long overhead_ticks;  // wall-clock ticks in dispatch around
+                                      // entry methods (decode, acquire, release)
+  unsigned long long pad;
 };
 
 struct PeState {

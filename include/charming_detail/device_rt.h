@@ -14,7 +14,9 @@
 //                        wait until head[dst] >= pos + slot - R     (relaxed)
 //   all threads:         write envelope + header + payload at pos
 //   each wave:           s_waitcnt 0; workgroup barrier
-//   producer (thread 0): ready[dst][pos/64 mod N] = pos/64 + 1     (release)
+//   producer (thread 0): ready[dst][pos/64 mod N] = pos/64 + 1     (release;
+//                        batched: one release covers every message an
+//                        entry method sent, see flush_publish)
 //   consumer (thread 0): poll ready[me][read/64 mod N] == read/64+1 (relaxed)
 //                        then an agent-scope acquire fence
 //   all threads:         run the entry method on the message in place
@@ -41,16 +43,22 @@ namespace rt {
 static __constant__ Ctx g_ctx;
 
 constexpr uint64_t kFail = ~0ull;
+constexpr int kMaxPending = 16;
 enum : int { CMD_EXIT = 0, CMD_INBOX = 1, CMD_REVIVED = 2 };
 
-// Workgroup-shared scratch used to broadcast thread 0's decisions.
+// Workgroup-shared scratch used to broadcast thread 0's decisions.  The
+// pending list holds publish words of messages already written but not yet
+// released: one L2 writeback then releases them all (see flush_publish).
 struct Lds {
   char* ptr;
   uint64_t pos;
+  uint64_t t_found;  // wall clock when the current message was found
   int cmd;
   int arg;
   int n;
-  int pad;
+  int n_pend;
+  uint64_t* pend_addr[kMaxPending];
+  uint64_t pend_tag[kMaxPending];
   uint32_t order[CHARMING_DEFERRED_MAX];
 };
 static __shared__ Lds s_lds;
@@ -172,6 +180,21 @@ __device__ void wg_copy(char* dst, const char* src, size_t n) {
 // send
 // ---------------------------------------------------------------------------
 
+// Thread 0: release every message this PE has written but not published.
+// A release at agent scope writes back the XCD's whole L2, so messages sent
+// by one entry method share one writeback instead of paying one each; the
+// list is flushed when an entry method (or main) returns, when it is full,
+// before any wait on another PE, and at grid barriers.
+__device__ void flush_publish() {
+  Lds& L = lds();
+  if (L.n_pend == 0) return;
+  release_fence();
+  for (int i = 0; i < L.n_pend; i++)
+    __hip_atomic_store(L.pend_addr[i], L.pend_tag[i], __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  L.n_pend = 0;
+}
+
 // Reserve `slot` bytes in dst's ring (thread 0).  Returns kFail when the
 // runtime is stopping or the request is invalid.
 __device__ uint64_t reserve(int dst, uint64_t slot) {
@@ -191,6 +214,7 @@ __device__ uint64_t reserve(int dst, uint64_t slot) {
   if (pos + slot - head0 > c.ring_bytes) {
     const uint64_t need = pos + slot - c.ring_bytes;
     uint64_t spins = 0;
+    flush_publish();  // never wait on another PE with messages held back
     while (ld_relaxed(&c.heads[(uint64_t)dst * 16]) < need) {
       if (dst == (int)blockIdx.x) {
         raise_error(ERR_SELF_RING_FULL, (uint32_t)slot);
@@ -233,8 +257,11 @@ __device__ void send_msg(int dst, msgtype type, int chare_id, int chare_idx,
   __syncthreads();
   if (threadIdx.x == 0) {
     const uint64_t g = pos / kGranule;
-    st_release(&c.ready[(uint64_t)dst * n_granules() + (g & (n_granules() - 1))],
-               g + 1);
+    if (L.n_pend == kMaxPending) flush_publish();
+    L.pend_addr[L.n_pend] =
+        &c.ready[(uint64_t)dst * n_granules() + (g & (n_granules() - 1))];
+    L.pend_tag[L.n_pend] = g + 1;
+    L.n_pend++;
     pe_state()->stats.sent++;
   }
 }
@@ -351,8 +378,13 @@ __device__ __noinline__ bool dispatch(envelope* env) {
       if (threadIdx.x == 0) raise_error(ERR_BAD_MESSAGE, (uint32_t)id);
       return true;
     }
-    return t.proxies[id]->call(m->chare_idx, m->ep_id,
-                               (char*)m + sizeof(regular_msg), m->refnum);
+    if (threadIdx.x == 0)
+      pe_state()->stats.overhead_ticks += wall_clock64() - lds().t_found;
+    const bool ok = t.proxies[id]->call(m->chare_idx, m->ep_id,
+                                        (char*)m + sizeof(regular_msg),
+                                        m->refnum);
+    if (threadIdx.x == 0) lds().t_found = wall_clock64();
+    return ok;
   }
   if (threadIdx.x == 0) raise_error(ERR_BAD_MESSAGE, (uint32_t)env->type);
   return true;
@@ -375,6 +407,7 @@ __device__ int poll_next() {
             (best < 0 || ps->deferred[i].seq < ps->deferred[best].seq))
           best = i;
       if (best >= 0) {
+        L.t_found = wall_clock64();
         ps->deferred[best].state = 3;  // being processed
         ps->n_revived--;
         L.arg = best;
@@ -385,6 +418,7 @@ __device__ int poll_next() {
     }
     const uint64_t g = ps->read_pos / kGranule;
     if (ld_relaxed(&rdy[g & (NG - 1)]) == g + 1) {
+      L.t_found = wall_clock64();
       fence_acquire();
       L.ptr = ring + (ps->read_pos & (c.ring_bytes - 1));
       return CMD_INBOX;
@@ -419,10 +453,12 @@ __device__ __noinline__ void process_inbox() {
   }
   __syncthreads();
   if (threadIdx.x == 0) {
+    flush_publish();
     wave_drain();
     ps->read_pos += slot;
     st_relaxed(&c.heads[(uint64_t)blockIdx.x * 16], ps->read_pos);
     if (ok) ps->stats.delivered++;
+    ps->stats.overhead_ticks += wall_clock64() - lds().t_found;
   }
 }
 
@@ -433,6 +469,7 @@ __device__ __noinline__ void process_revived() {
   const bool ok = dispatch(env);
   __syncthreads();
   if (threadIdx.x == 0) {
+    flush_publish();
     DeferredEntry& d = ps->deferred[e];
     if (ok) {
       d.state = 0;
@@ -469,6 +506,7 @@ __device__ __noinline__ void pe_loop() {
 __device__ void grid_barrier() {
   __syncthreads();
   if (threadIdx.x == 0) {
+    flush_publish();
     uint32_t* cnt = &g_ctx.ctl[CTL_BAR_COUNT];
     uint32_t* gen = &g_ctx.ctl[CTL_BAR_GEN];
     const uint32_t g = ld_relaxed(gen);
@@ -509,6 +547,7 @@ __device__ void pe_begin() {
     ps->n_revived = 0;
     ps->ended = 0;
     ps->hw = 0;
+    lds().n_pend = 0;
     for (int i = 0; i < CHARMING_DEFERRED_MAX; i++) ps->deferred[i].state = 0;
     ps->stats = Stats{};
   }
@@ -579,6 +618,12 @@ __device__ int device_atoi(const char* str, int strlen) {
   return tmp;
 }
 
+__device__ void flush_sends() {
+  __syncthreads();
+  if (threadIdx.x == 0) rt::flush_publish();
+  __syncthreads();
+}
+
 __device__ double wall_time() {
   return (double)wall_clock64() / rt::g_ctx.wall_hz;
 }
@@ -597,6 +642,8 @@ namespace rt {
 // (128 VGPRs), and the kernel must not add registers of its own on top.
 __device__ __noinline__ void run_user_main() {
   charm::main(g_ctx.argc, g_ctx.argv, g_ctx.argvs, (int)blockIdx.x);
+  __syncthreads();
+  if (threadIdx.x == 0) flush_publish();
 }
 
 __device__ __noinline__ void pe_run() {

@@ -54,6 +54,8 @@ typedef struct chm_jacobi_result {
   int launches;          /* kernel launches issued by the solve */
   long long messages;    /* messages delivered (persistent form) */
   long long deferred;    /* refnum-mismatched messages deferred (persistent) */
+  double sched_overhead_ms; /* persistent: mean per PE in the scheduler around
+                               entry methods (decode, fences, release) */
 } chm_jacobi_result;
 
 /* Tuning of the sweep: chunks of 128 columns per wave panel and rows kept in
@@ -63,6 +65,8 @@ typedef struct chm_jacobi_tuning {
   int rows_in_flight;/* 1, 2, 4 or 8 */
   int band_rows;     /* rows per wave band (plain form) */
   int wg_waves;      /* waves per workgroup (plain form) */
+  int overlap;       /* persistent form: 1 = sweep the edges first and send the
+                        halos before the interior (default 0) */
 } chm_jacobi_tuning;
 
 /* Allocate the grids of one problem in HBM for the given form and fill u_0.
@@ -90,6 +94,13 @@ int chm_jacobi2d_read_grid(void* handle, double* out, size_t out_elems);
  * coordinates) of the current grid, computed on the device. */
 int chm_jacobi2d_hash_region(void* handle, int r0, int r1, int c0, int c1,
                              uint64_t* hash);
+
+/* Per-chare time of the last persistent solve, in ms: mean over chares of
+ * {sweep, send, wait-for-halos, inside halo-receive entry methods} in
+ * out8[0..3], maximum in out8[4..7].  The reference keeps the same
+ * per-chare phase timers and prints them (examples/jacobi2d/jacobi2d.cu:
+ * 148-166, 224-262). */
+int chm_jacobi2d_phase_stats(void* handle, double* out8);
 
 int chm_jacobi2d_destroy(void* handle);
 
