@@ -18,7 +18,9 @@ FORM_PLAIN, FORM_GRAPH, FORM_PERSISTENT = 0, 1, 2
 INIT_SEEDED, INIT_REFERENCE = 0, 1
 
 STATUS = {0: "OK", -1: "INVALID", -2: "HIP", -3: "NOMEM", -4: "UNSUPPORTED",
-          -5: "DEVICE", -6: "TIMEOUT", -7: "STATE", -8: "NOGPU"}
+          -5: "DEVICE", -6: "TIMEOUT", -7: "STATE", -8: "NOGPU", -9: "BUSY"}
+E_BUSY = -9
+SERVE_ENGINE, SERVE_PLAIN = 0, 1
 
 
 class ChmError(RuntimeError):
@@ -60,6 +62,49 @@ class Tuning(C.Structure):
                 ("overlap", C.c_int)]
 
 
+class BatchProblem(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("iters", C.c_int),
+                ("block", C.c_int), ("tol", C.c_double), ("seed", C.c_uint64),
+                ("init_mode", C.c_int), ("boundary", C.c_double)]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("wg_threads", C.c_int), ("block", C.c_int),
+                ("panel_chunks", C.c_int), ("rows_in_flight", C.c_int),
+                ("timeout_s", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class BatchResult(C.Structure):
+    _fields_ = [("device_ms", C.c_double), ("tasks", C.c_longlong),
+                ("steals", C.c_longlong), ("idle_polls", C.c_longlong),
+                ("cell_updates", C.c_longlong), ("n_wg", C.c_int),
+                ("wg_threads", C.c_int)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("mode", C.c_int), ("budget_bytes", C.c_size_t),
+                ("max_active", C.c_int), ("wg_threads", C.c_int),
+                ("block", C.c_int), ("panel_chunks", C.c_int),
+                ("rows_in_flight", C.c_int)]
+
+
+class ServerCompletion(C.Structure):
+    _fields_ = [("ticket", C.c_uint64), ("iters", C.c_int),
+                ("hash", C.c_uint64), ("residual_lo", C.c_uint64),
+                ("residual_hi", C.c_uint64), ("residual0_lo", C.c_uint64),
+                ("residual0_hi", C.c_uint64), ("device_ms", C.c_double),
+                ("latency_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("active", C.c_int), ("bytes_in_use", C.c_size_t),
+                ("budget_bytes", C.c_size_t), ("submitted", C.c_longlong),
+                ("completed", C.c_longlong)]
+
+
 _jac = None
 
 
@@ -77,6 +122,16 @@ def jacobi_lib():
                                                C.c_int, C.POINTER(C.c_uint64)]
         L.chm_jacobi2d_destroy.argtypes = [vp]
         L.chm_jacobi2d_phase_stats.argtypes = [vp, C.c_void_p]
+        BP, BC, BR = C.POINTER(BatchProblem), C.POINTER(BatchConfig), C.POINTER(BatchResult)
+        L.chm_jacobi2d_batch_run.argtypes = [BP, C.c_int, BC, BR, vp, vp, vp]
+        L.chm_jacobi2d_batch_plain.argtypes = [BP, C.c_int, C.c_int, BR, vp, vp, vp]
+        L.chm_jacobi2d_batch_autotune.argtypes = [BP, C.c_int, C.c_int, BC, vp,
+                                                  C.POINTER(C.c_double)]
+        L.chm_jacobi2d_server_start.argtypes = [C.POINTER(ServerConfig), C.POINTER(vp)]
+        L.chm_jacobi2d_server_submit.argtypes = [vp, BP, C.POINTER(C.c_uint64)]
+        L.chm_jacobi2d_server_poll.argtypes = [vp, C.POINTER(ServerCompletion), C.c_int]
+        L.chm_jacobi2d_server_stats.argtypes = [vp, C.POINTER(ServerStats)]
+        L.chm_jacobi2d_server_stop.argtypes = [vp, C.c_int, C.c_double]
         L.chm_jacobi2d_last_error.restype = C.c_char_p
         L.chm_jacobi2d_abi_version.restype = C.c_int
         _jac = L
@@ -159,3 +214,124 @@ class Jacobi2D:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------------
+# batched and served forms
+# ---------------------------------------------------------------------------
+def _problems(probs):
+    """A ctypes array of BatchProblem from dicts (width, height, iters and
+    optionally block, tol, seed, init_mode, boundary)."""
+    arr = (BatchProblem * len(probs))()
+    for i, p in enumerate(probs):
+        arr[i] = BatchProblem(p["width"], p["height"], p["iters"],
+                              p.get("block", 0), p.get("tol", 0.0),
+                              p.get("seed", 0), p.get("init_mode", INIT_SEEDED),
+                              p.get("boundary", 1.0))
+    return arr
+
+
+def _batch_outputs(n):
+    return (np.zeros(n, np.uint64), np.zeros(2 * n, np.uint64),
+            np.zeros(n, np.int32))
+
+
+def _pack(res, hashes, residuals, iters):
+    res_int = [int(residuals[2 * i]) | (int(residuals[2 * i + 1]) << 64)
+               for i in range(len(hashes))]
+    return {"result": res, "hashes": [int(h) for h in hashes],
+            "residuals": res_int, "iters": iters.tolist()}
+
+
+def batch_run(probs, wg_threads=0, block=0, panel_chunks=0, rows_in_flight=0,
+              timeout_s=0.0):
+    """Solve every problem in one persistent engine launch."""
+    arr = _problems(probs)
+    cfg = BatchConfig(wg_threads, block, panel_chunks, rows_in_flight, timeout_s)
+    res = BatchResult()
+    h, r, it = _batch_outputs(len(probs))
+    _check(jacobi_lib().chm_jacobi2d_batch_run(
+        arr, len(probs), C.byref(cfg), C.byref(res), h.ctypes.data,
+        r.ctypes.data, it.ctypes.data))
+    return _pack(res, h, r, it)
+
+
+def batch_plain(probs, use_graph=False):
+    """The same problems one after another, one launch per sweep (or one
+    hipGraph per problem)."""
+    arr = _problems(probs)
+    res = BatchResult()
+    h, r, it = _batch_outputs(len(probs))
+    _check(jacobi_lib().chm_jacobi2d_batch_plain(
+        arr, len(probs), 1 if use_graph else 0, C.byref(res), h.ctypes.data,
+        r.ctypes.data, it.ctypes.data))
+    return _pack(res, h, r, it)
+
+
+def batch_autotune(probs, sample_iters=20):
+    """Returns (config dict, per-problem tile sides, cell updates/s)."""
+    arr = _problems(probs)
+    cfg = BatchConfig()
+    blocks = np.zeros(len(probs), np.int32)
+    rate = C.c_double()
+    _check(jacobi_lib().chm_jacobi2d_batch_autotune(
+        arr, len(probs), sample_iters, C.byref(cfg), blocks.ctypes.data,
+        C.byref(rate)))
+    return cfg.as_dict(), blocks.tolist(), rate.value
+
+
+class Server:
+    """A Jacobi2D solve server: requests are admitted against an HBM budget
+    and solved in the persistent engine (mode 'engine', continuous batching)
+    or one after another in the plain per-launch form (mode 'plain')."""
+
+    def __init__(self, mode="engine", budget_bytes=16 << 30, max_active=0,
+                 wg_threads=0, block=0, panel_chunks=0, rows_in_flight=0):
+        self.mode = mode
+        cfg = ServerConfig(SERVE_ENGINE if mode == "engine" else SERVE_PLAIN,
+                           budget_bytes, max_active, wg_threads, block,
+                           panel_chunks, rows_in_flight)
+        h = C.c_void_p()
+        _check(jacobi_lib().chm_jacobi2d_server_start(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self._buf = (ServerCompletion * 256)()
+
+    def submit(self, prob):
+        """Ticket, or None when admission is refused for now (BUSY)."""
+        arr = _problems([prob])
+        t = C.c_uint64()
+        rc = jacobi_lib().chm_jacobi2d_server_submit(self.h, arr, C.byref(t))
+        if rc == E_BUSY:
+            return None
+        _check(rc)
+        return t.value
+
+    def poll(self):
+        n = jacobi_lib().chm_jacobi2d_server_poll(self.h, self._buf, 256)
+        if n < 0:
+            _check(n)
+        out = []
+        for i in range(n):
+            c = self._buf[i]
+            out.append({"ticket": c.ticket, "iters": c.iters, "hash": c.hash,
+                        "residual": c.residual_lo | (c.residual_hi << 64),
+                        "residual0": c.residual0_lo | (c.residual0_hi << 64),
+                        "device_ms": c.device_ms, "latency_ms": c.latency_ms})
+        return out
+
+    def stats(self):
+        s = ServerStats()
+        _check(jacobi_lib().chm_jacobi2d_server_stats(self.h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in s._fields_}
+
+    def stop(self, abort=False, timeout_s=120.0):
+        if getattr(self, "h", None):
+            h, self.h = self.h, None
+            _check(jacobi_lib().chm_jacobi2d_server_stop(h, 1 if abort else 0,
+                                                         timeout_s))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.stop(abort=a[0] is not None)

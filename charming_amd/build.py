@@ -32,7 +32,8 @@ DEVICE_FLAGS = [
 # (library name, sources, extra link libraries from this build)
 LIBS = [
     ("libcharming_amd.so", ["runtime/host_runtime.hip"], []),
-    ("libchm_jacobi2d.so", ["jacobi2d/plain.hip", "jacobi2d/persistent.hip"],
+    ("libchm_jacobi2d.so", ["jacobi2d/plain.hip", "jacobi2d/persistent.hip",
+                           "jacobi2d/batch.hip"],
      ["charming_amd"]),
     # the example chare programs as loadable libraries (run through chm_main)
     ("libchm_hello.so", ["examples/hello.hip"], ["charming_amd"]),

@@ -1,0 +1,114 @@
+// Shared (host + device) definitions of the Jacobi2D task engine: the
+// batched and served forms of the solver.
+//
+// Many independent Jacobi2D problems share one persistent launch.  Each
+// problem is cut into square chare tiles (side chosen per problem); one task
+// is one sweep of one tile.  A tile's completion is the message that
+// releases its neighbours: per tile and per iteration parity, a counter of
+// unmet dependencies (the tile's own previous sweep, each neighbour's
+// previous sweep, and - when the problem stops on a residual - the stop
+// decision two sweeps back) is decremented by every producer; whoever takes
+// it to zero enqueues the task.  Halos are read in place from the
+// neighbour's cells (the zero-copy halo mode: the message carries no
+// payload).  Ready tasks go to the completing workgroup's own queue; idle
+// workgroups steal from the others.
+#pragma once
+#include <stdint.h>
+
+namespace chm {
+namespace eng {
+
+constexpr uint32_t kMaxSlots = 4096;        // problems in flight
+constexpr uint32_t kMaxTiles = 1u << 18;    // tiles per problem
+enum : uint32_t { K_INIT = 0, K_ITER = 1, K_HASH = 2 };
+
+__host__ __device__ inline uint32_t task_word(uint32_t kind, uint32_t slot,
+                                              uint32_t tile) {
+  return (kind << 30) | (slot << 18) | tile;
+}
+__host__ __device__ inline uint32_t task_kind(uint32_t w) { return w >> 30; }
+__host__ __device__ inline uint32_t task_slot(uint32_t w) {
+  return (w >> 18) & (kMaxSlots - 1);
+}
+__host__ __device__ inline uint32_t task_tile(uint32_t w) {
+  return w & (kMaxTiles - 1);
+}
+
+// A request as the host publishes it.
+struct Request {
+  double* a;          // u_k for even k
+  double* b;          // u_k for odd k
+  uint32_t* pend;     // 3 words per tile: dependency counters of the next
+                      // even and odd sweep, sweeps done
+  int64_t pitch;
+  int W, H, bs, tr, tc, max_iters;
+  int init_on_device; // 1: the engine seeds the grids (init tasks)
+  int init_mode;
+  double boundary;
+  double tol2;        // < 0: run exactly max_iters sweeps
+  uint64_t seed;
+  uint64_t ticket;
+  uint32_t slot;
+  uint32_t pad;
+};
+
+// Device state of a problem slot (HBM, written only by the engine).
+struct Prob {
+  Request r;
+  uint32_t inflight;      // tasks enqueued and not finished
+  uint32_t state;         // 0 running, 1 stopped, 2 hashing
+  uint32_t stop_iters;    // sweeps in the final result
+  uint32_t pad1;
+  uint32_t iter_tiles[4]; // tiles that finished sweep k (slot k & 3)
+  uint32_t pad0[4];
+  uint64_t res[4][2];     // residual partials of sweep k (slot k & 3)
+  uint64_t s0[2];         // residual of the first sweep
+  uint64_t s_last[2];     // residual of the last sweep
+  uint64_t hash;
+  uint64_t t_admit, t_done;  // device wall clock
+};
+
+// A completion record (host-coherent memory); `valid` = ticket + 1 is
+// written last.
+struct Completion {
+  uint64_t ticket;
+  uint64_t hash;
+  uint64_t s_last_lo, s_last_hi;
+  uint64_t s0_lo, s0_hi;
+  uint64_t t_admit, t_done;  // device wall-clock ticks
+  uint32_t iters;
+  uint32_t slot;
+  uint64_t valid;
+};
+
+// Device control words, one 128-byte line each.
+enum : int {
+  CW_REQ_HEAD = 0,   // requests claimed
+  CW_REQ_TAIL = 16,  // requests published (mirror of the host word)
+  CW_STOP = 32,      // 1: exit, 2: abort (mirror of the host word)
+  CW_DONE = 48,      // completions written
+  CW_ACTIVE = 64,    // problems admitted and not done
+  CW_STATS = 80,     // [0] tasks [1] steals [2] idle polls [3] cell updates
+  CW_WORDS = 96
+};
+
+struct EngineDesc {
+  Prob* probs;              // kMaxSlots
+  uint64_t* qcells;         // n_wg x qcap
+  uint64_t* qhead;          // n_wg, stride 16
+  uint64_t* qtail;          // n_wg, stride 16
+  uint32_t qcap;            // power of two
+  int n_wg;
+  const Request* reqs;      // request ring (req_cap entries)
+  uint32_t req_cap;         // power of two
+  int n_pollers;            // workgroups that poll the host words
+  volatile uint64_t* host_tail;  // host-written request count (or null)
+  volatile uint32_t* host_stop;  // host-written stop word (or null)
+  Completion* dones;        // host-coherent completion ring (req_cap entries)
+  uint64_t* ctl;            // CW_WORDS device control words
+  uint64_t expected;        // batch mode: exit after this many completions
+  int nc, u;                // sweep tile shape
+};
+
+}  // namespace eng
+}  // namespace chm

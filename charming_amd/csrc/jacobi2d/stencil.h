@@ -126,15 +126,37 @@ struct EdgeOut {
 typedef __attribute__((address_space(1))) const double gcdouble;
 typedef __attribute__((address_space(1))) double gdouble;
 typedef double v2d __attribute__((ext_vector_type(2)));
+#ifndef CHM_NT_LOADS
+#define CHM_NT_LOADS 0
+#endif
+#ifndef CHM_NT_STORES
+#define CHM_NT_STORES 1
+#endif
+// Streaming accesses: every input row is read once per sweep and every
+// output row written once, so stores carry the non-temporal hint: they
+// leave few dirty lines in L2, which keeps the L2 writeback of a message
+// release short.
 __device__ inline double2 gload2(const gcdouble* p) {
-  const v2d v = *(const __attribute__((address_space(1))) v2d*)p;
+  const __attribute__((address_space(1))) v2d* q =
+      (const __attribute__((address_space(1))) v2d*)p;
+#if CHM_NT_LOADS
+  const v2d v = __builtin_nontemporal_load(q);
+#else
+  const v2d v = *q;
+#endif
   return make_double2(v.x, v.y);
 }
 __device__ inline void gstore2(gdouble* p, double a, double b) {
   v2d v;
   v.x = a;
   v.y = b;
-  *(__attribute__((address_space(1))) v2d*)p = v;
+  __attribute__((address_space(1))) v2d* q =
+      (__attribute__((address_space(1))) v2d*)p;
+#if CHM_NT_STORES
+  __builtin_nontemporal_store(v, q);
+#else
+  *q = v;
+#endif
 }
 
 template <int NC>
@@ -150,17 +172,19 @@ struct Panel {
   int64_t pitch;
   int w, h;    // interior of the array
   int c0;      // first interior column of the panel (1-based)
+  int cend;    // last column this panel may write (<= w)
   int lane;
   int ecol;    // column this lane loads into RowRegs::e (-1: none)
 
   __device__ inline void setup(const double* in_, double* out_, int64_t p,
-                               int w_, int h_, int c0_) {
+                               int w_, int h_, int c0_, int cend_) {
     in = (const gcdouble*)in_;
     out = (gdouble*)out_;
     pitch = p;
     w = w_;
     h = h_;
     c0 = c0_;
+    cend = cend_;
     lane = __lane_id();
     ecol = -1;
     if (lane == 0) ecol = c0 - 1;
@@ -198,7 +222,7 @@ struct Panel {
       const double na = (a + bl + b + up.v[k].x + dn.v[k].x) * 0.2;
       const double nb = (b + a + ar + up.v[k].y + dn.v[k].y) * 0.2;
       const int c = col(k);
-      const bool two = c + 1 <= w, one = c == w;
+      const bool two = c + 1 <= cend, one = c == cend;
       if (two) {
         gstore2(orow + c, na, nb);
         if (RES) {
@@ -229,7 +253,8 @@ struct Panel {
 };
 
 // Sweep rows [r0, r1) (1-based, r1 <= h+1) of the panel starting at column
-// c0.  Reads rows r0-1 .. r1 of `in`, writes rows r0 .. r1-1 of `out`.
+// c0.  Reads rows r0-1 .. r1 of `in`, writes rows r0 .. r1-1 of `out`,
+// columns c0 .. min(c0 + 128*NC - 1, cend) (cend <= w; default w).
 // Registers hold a ring of U+2 rows: computing row r frees row r-1's slot,
 // which immediately receives the load of row r+U+1, so U rows stay in
 // flight.  The row loop is unrolled by the ring length so every slot index
@@ -238,11 +263,12 @@ template <int NC, int U, bool RES, bool PACK>
 __device__ inline void sweep_panel(const double* __restrict__ in,
                                    double* __restrict__ out, int64_t pitch,
                                    int w, int h, int c0, int r0, int r1,
-                                   u128acc& acc, const EdgeOut& eo) {
+                                   u128acc& acc, const EdgeOut& eo,
+                                   int cend = -1) {
   if (r0 >= r1) return;
   constexpr int K = U + 2;
   Panel<NC> p;
-  p.setup(in, out, pitch, w, h, c0);
+  p.setup(in, out, pitch, w, h, c0, cend < 0 ? w : cend);
   RowRegs<NC> R[K];
 #pragma unroll
   for (int s = 0; s < K; s++)

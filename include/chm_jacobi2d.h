@@ -114,6 +114,121 @@ int chm_hbm_copy_probe(size_t bytes, int reps, double* ms_per_copy);
 /* Sanity hook with no GPU work: returns the ABI version. */
 int chm_jacobi2d_abi_version(void);
 
+/* ------------------------------------------------------------------------
+ * Batched and served forms.  No reference counterpart: the reference runs
+ * one Jacobi2D problem per job (examples/jacobi2d/jacobi2d.cu:26-77).  Here
+ * many independent problems share one persistent launch of a task engine:
+ * each problem is cut into square chare tiles, one task is one sweep of one
+ * tile, a tile's completion is the (zero-copy) halo message that releases
+ * its neighbours, and idle workgroups steal ready tasks from the others.
+ * Every problem's result is bit-identical to the same problem solved alone
+ * (and to the monolithic CPU sweep).
+ * ------------------------------------------------------------------------ */
+
+typedef struct chm_batch_problem {
+  int width;       /* interior columns W */
+  int height;      /* interior rows H */
+  int iters;       /* sweeps (the cap when tol > 0) */
+  int block;       /* chare tile side; 0 = the configuration's choice */
+  double tol;      /* > 0: stop after the first sweep k whose residual
+                      S_k <= tol^2 * S_0 (relative L2 change <= tol) */
+  uint64_t seed;   /* CHM_INIT_SEEDED */
+  int init_mode;   /* enum chm_jacobi_init */
+  double boundary; /* ring value for CHM_INIT_SEEDED */
+} chm_batch_problem;
+
+typedef struct chm_batch_config {
+  int wg_threads;     /* 256, 512 or 1024 (0 = 1024): workgroups per CU =
+                         1024 / wg_threads */
+  int block;          /* tile side for problems with block 0 (0 = by size) */
+  int panel_chunks;   /* sweep shape: 1 or 2 (0 = 1) */
+  int rows_in_flight; /* sweep shape: 2 or 4 (0 = 2) */
+  double timeout_s;   /* watchdog (0 = 120 s) */
+} chm_batch_config;
+
+typedef struct chm_batch_result {
+  double device_ms;       /* the engine launch (or the plain sequence) */
+  long long tasks;        /* tile tasks executed */
+  long long steals;       /* tasks taken from another workgroup's queue */
+  long long idle_polls;   /* idle poll rounds (x64) */
+  long long cell_updates; /* interior cells written by sweeps */
+  int n_wg, wg_threads;
+} chm_batch_result;
+
+/* Solve n problems in one persistent launch; grids are seeded in HBM before
+ * the timed launch.  Per problem i: hashes[i] = hash of the interior of the
+ * result, residuals[2i..2i+1] = exact residual of the last sweep (lo, hi),
+ * iters_done[i].  Output arrays may be NULL. */
+int chm_jacobi2d_batch_run(const chm_batch_problem* probs, int n,
+                           const chm_batch_config* cfg, chm_batch_result* res,
+                           uint64_t* hashes, uint64_t* residuals,
+                           int* iters_done);
+
+/* The same problems solved one after another in the plain per-iteration
+ * form (use_graph = 0: one launch per sweep; 1: each problem's sweeps
+ * captured beforehand as a hipGraph and replayed).  Same outputs. */
+int chm_jacobi2d_batch_plain(const chm_batch_problem* probs, int n,
+                             int use_graph, chm_batch_result* res,
+                             uint64_t* hashes, uint64_t* residuals,
+                             int* iters_done);
+
+/* Autotuner: picks the workgroup size and sweep shape, and a tile side per
+ * problem from {64, 128, 256, 512} (blocks_out[i]), by coordinate descent
+ * over size classes, timing short engine runs of the whole batch capped at
+ * sample_iters sweeps.  best_rate = cell updates per second of the best. */
+int chm_jacobi2d_batch_autotune(const chm_batch_problem* probs, int n,
+                                int sample_iters, chm_batch_config* best,
+                                int* blocks_out, double* best_rate);
+
+enum chm_server_mode {
+  CHM_SERVE_ENGINE = 0, /* continuous batching in the persistent engine */
+  CHM_SERVE_PLAIN = 1   /* one request at a time, one launch per sweep */
+};
+
+typedef struct chm_server_config {
+  int mode;            /* enum chm_server_mode */
+  size_t budget_bytes; /* HBM for resident grids (0 = 16 GiB) */
+  int max_active;      /* requests resident at once (0 = 4096) */
+  int wg_threads, block, panel_chunks, rows_in_flight; /* as chm_batch_config */
+} chm_server_config;
+
+typedef struct chm_server_completion {
+  uint64_t ticket;
+  int iters;             /* sweeps in the result */
+  uint64_t hash;         /* hash of the interior of the result */
+  uint64_t residual_lo, residual_hi;   /* S of the last sweep */
+  uint64_t residual0_lo, residual0_hi; /* S_0 (tol > 0) */
+  double device_ms;      /* admission to completion on the device clock */
+  double latency_ms;     /* submit to poll on the host clock */
+} chm_server_completion;
+
+typedef struct chm_server_stats {
+  int active;
+  size_t bytes_in_use, budget_bytes;
+  long long submitted, completed;
+} chm_server_stats;
+
+/* Start a server: allocates the grid pool (budget) once, launches the engine
+ * (mode ENGINE) or a host worker thread (mode PLAIN). */
+int chm_jacobi2d_server_start(const chm_server_config* cfg, void** server);
+
+/* Admit one request: CHM_OK and *ticket, or CHM_E_BUSY when its grids do not
+ * fit the free budget or max_active requests are resident; the caller keeps
+ * it and submits again after completions free memory.  Grids are seeded on
+ * the device. */
+int chm_jacobi2d_server_submit(void* server, const chm_batch_problem* p,
+                               uint64_t* ticket);
+
+/* Collect up to max_out completions (non-blocking); returns the count. */
+int chm_jacobi2d_server_poll(void* server, chm_server_completion* out,
+                             int max_out);
+
+int chm_jacobi2d_server_stats(void* server, chm_server_stats* out);
+
+/* Wait for every admitted request to complete (up to timeout_s), then stop
+ * the engine and free the pool.  abort_now = 1 stops at once. */
+int chm_jacobi2d_server_stop(void* server, int abort_now, double timeout_s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,5 +17,6 @@
 #define CHM_E_TIMEOUT (-6)     /* host watchdog expired; the kernel was told to stop */
 #define CHM_E_STATE (-7)       /* call out of order (e.g. run before init) */
 #define CHM_E_NOGPU (-8)       /* no usable gfx950 device */
+#define CHM_E_BUSY (-9)        /* admission refused for now: budget or slots in use */
 
 #endif /* CHM_STATUS_H_ */
