@@ -13,6 +13,16 @@ the plain per-launch and hipGraph forms of the same sweep, the measured copy
 ceiling, the roofline of the dominant kernel (16 algorithmic bytes per
 cell-update over its HIP-event duration) and the CPU baseline (the oracle's
 threaded message-driven runner on the host cores, bounded sample).
+
+Two more blocks, each bit-checked across forms:
+  "batch": 512 uneven problems (sides log-uniform 64..4096, half square and
+           half 2:1, 50..500 sweeps) autotuned and solved in one persistent
+           engine launch, against the same problems one after another in the
+           plain per-launch form and as one hipGraph per problem;
+  "serve": Poisson request streams (sides log-uniform 256..8192, stop at
+           relative residual 1e-6 or 2000 sweeps, 200 GB budget) served by
+           the engine (continuous batching) and by the plain per-request
+           form: solves/s and p50/p99 latency per arrival rate.
 """
 import argparse
 import json
@@ -77,6 +87,78 @@ def cpu_baseline(budget_s):
                       f"threads; iteration phase timed ({t:.1f} s)"}
 
 
+def bench_batch(cam, sample_iters):
+    """512 uneven problems: autotuned engine launch vs plain and graph."""
+    from charming_amd import workloads as WL
+    probs = WL.batch_problems()
+    t0 = time.perf_counter()
+    cfg, blocks, rate = cam.batch_autotune(probs, sample_iters=sample_iters)
+    tune_s = time.perf_counter() - t0
+    tuned = [dict(p, block=b) for p, b in zip(probs, blocks)]
+    kw = {k: cfg[k] for k in ("wg_threads", "panel_chunks", "rows_in_flight")}
+    eng = cam.batch_run(tuned, **kw)
+    eng2 = cam.batch_run(tuned, **kw)
+    plain = cam.batch_plain(probs)
+    graph = cam.batch_plain(probs, use_graph=True)
+    same = (eng["hashes"] == plain["hashes"] == graph["hashes"] == eng2["hashes"]
+            and eng["residuals"] == plain["residuals"] == graph["residuals"])
+    cells = WL.cell_updates(probs)
+
+    def rate_of(o):
+        ms = o["result"].device_ms
+        return {"ms": round(ms, 2), "gcell_per_s": round(cells / ms / 1e6, 2),
+                "solves_per_s": round(len(probs) / ms * 1e3, 1)}
+    e = rate_of(min(eng, eng2, key=lambda o: o["result"].device_ms))
+    r = eng["result"]
+    bcount = {b: blocks.count(b) for b in sorted(set(blocks))}
+    return {
+        "workload": "512 problems, sides log-uniform 64..4096 (even index "
+                    "square, odd 2:1), sweeps uniform 50..500, seeded",
+        "cell_updates": cells,
+        "engine": dict(e, tasks=r.tasks, steals=r.steals, n_wg=r.n_wg,
+                       wg_threads=r.wg_threads),
+        "plain": rate_of(plain), "graph": rate_of(graph),
+        "speedup_vs_plain": round(plain["result"].device_ms /
+                                  min(eng["result"].device_ms,
+                                      eng2["result"].device_ms), 2),
+        "speedup_vs_graph": round(graph["result"].device_ms /
+                                  min(eng["result"].device_ms,
+                                      eng2["result"].device_ms), 2),
+        "autotune": {"config": kw, "tiles": bcount, "sample_iters": sample_iters,
+                     "sample_gcell_per_s": round(rate / 1e9, 2),
+                     "seconds": round(tune_s, 1)},
+        "bit_identical_across_forms": same,
+    }
+
+
+def bench_serve(cam, rates, n_req, budget):
+    """Poisson request streams, engine (continuous batching) vs plain."""
+    from charming_amd import serve as SV
+    from charming_amd import workloads as WL
+    out = {"workload": f"{n_req} requests per rate, sides log-uniform "
+                       "256..8192 per side, stop at relative residual 1e-6 "
+                       "or 2000 sweeps",
+           "budget_bytes": budget, "rates": []}
+    same = True
+    for rate in rates:
+        arr, reqs = WL.serve_requests(n_req, rate)
+        row = {"offered_rate": rate}
+        res = {}
+        for mode in ("engine", "plain"):
+            r = SV.run_stream(mode, arr, reqs, budget_bytes=budget)
+            res[mode] = r["results"]
+            row[mode] = r["summary"]
+        same = same and all(a["hash"] == b["hash"] and a["iters"] == b["iters"]
+                            for a, b in zip(res["engine"], res["plain"]))
+        row["solves_ratio"] = round(row["engine"]["solves_per_s"] /
+                                    row["plain"]["solves_per_s"], 2)
+        out["rates"].append(row)
+        print(json.dumps({"serve_rate": rate, "engine": row["engine"],
+                          "plain": row["plain"]}), file=sys.stderr, flush=True)
+    out["bit_identical_engine_vs_plain"] = same
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +167,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--tune-iters", type=int, default=20,
+                    help="sweeps per problem in the autotuner's samples")
+    ap.add_argument("--serve-rates", default="10,100,2000")
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--calibrate", action="store_true",
                     help="only run the streaming copy of a known size (1 GiB "
                          "read + 1 GiB written per launch) used to scale "
@@ -186,6 +275,12 @@ def main():
                           "launches_per_solve": ITERS}
         out["compare"] = comp
 
+    if not args.no_batch:
+        out["batch"] = bench_batch(cam, args.tune_iters)
+    if not args.no_serve:
+        rates = [float(x) for x in args.serve_rates.split(",") if x]
+        out["serve"] = bench_serve(cam, rates, args.serve_requests,
+                                   int(args.serve_budget_gb * 1e9))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_budget)
     print(json.dumps(out), flush=True)

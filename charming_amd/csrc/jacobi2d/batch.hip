@@ -126,8 +126,15 @@ __device__ void release_deps(const EngineDesc& E, Prob& p, const Request& r,
 
 // Thread 0, after sweep k of `tile`: release sweep k+1 of the tile and its
 // neighbours; the last tile of a sweep whose residual matters decides
-// whether the problem stops.  Returns F_GATE when the decision lets sweep
-// k+2 of every tile go (its gate is then dropped by the whole workgroup).
+// whether the problem stops.  Returns F_GATE when the decision lets later
+// sweeps of every tile go (their gates are then dropped by the whole
+// workgroup).
+//
+// Decisions on sweeps k and k+1 can finish in either order (sweep k+1 only
+// waits for the decision on k-1), so a stop keeps the smallest sweep count
+// (atomic min) and leaves that sweep's residual slot untouched; the result
+// u_{k+1} is intact because sweep k+1 writes the other array and sweep k+2
+// is never released.
 __device__ int complete_iter(const EngineDesc& E, Prob& p, const Request& r,
                              uint32_t slot, uint32_t tile, int k) {
   if (k + 1 < r.max_iters) release_deps(E, p, r, slot, tile, (k + 1) & 1);
@@ -138,30 +145,30 @@ __device__ int complete_iter(const EngineDesc& E, Prob& p, const Request& r,
   const int sk = tol ? (k & 3) : 0;
   if (aadd(&p.iter_tiles[sk], 1u) != n_tiles - 1) return 0;
   // every tile has finished sweep k: its residual is complete
-  ast(&p.iter_tiles[sk], 0u);
   const uint64_t lo = ald(&p.res[sk][0]), hi = ald(&p.res[sk][1]);
-  ast(&p.res[sk][0], 0ull);
-  ast(&p.res[sk][1], 0ull);
   uint64_t s0lo, s0hi;
   if (k == 0) {
     s0lo = lo;
     s0hi = hi;
     ast(&p.s0[0], lo);
     ast(&p.s0[1], hi);
-  } else {
+  } else {  // sweep 1 waited for the decision on sweep 0
     s0lo = ald(&p.s0[0]);
     s0hi = ald(&p.s0[1]);
   }
   // the stop rule of the monolithic tolerance solve, in the same doubles
   const bool stop = last || u128d(lo, hi) <= r.tol2 * u128d(s0lo, s0hi);
   if (stop) {
-    ast(&p.s_last[0], lo);
-    ast(&p.s_last[1], hi);
-    ast(&p.stop_iters, (uint32_t)(k + 1));
+    __hip_atomic_fetch_min(&p.stop_iters, (uint32_t)(k + 1), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
     ast(&p.state, 1u);
+  } else {
+    ast(&p.iter_tiles[sk], 0u);
+    ast(&p.res[sk][0], 0ull);
+    ast(&p.res[sk][1], 0ull);
   }
   release_agent();
-  return (!stop && tol && k + 2 < r.max_iters) ? F_GATE : 0;
+  return (!stop && tol) ? F_GATE : 0;
 }
 
 // Thread 0: a task of the problem is finished.  The last task of a stopped
@@ -187,14 +194,16 @@ __device__ void publish(const EngineDesc& E, Prob& p, uint32_t slot) {
   const uint64_t idx = aadd(&E.ctl[CW_DONE], 1ull);
   Completion* c = &E.dones[idx & (E.req_cap - 1)];
   const uint64_t ticket = p.r.ticket;
-  const uint64_t v[8] = {ticket,           ald(&p.hash),     ald(&p.s_last[0]),
-                         ald(&p.s_last[1]), ald(&p.s0[0]),    ald(&p.s0[1]),
+  const uint32_t iters = ald(&p.stop_iters);
+  const int sk = p.r.tol2 >= 0 ? (int)((iters - 1) & 3) : 0;
+  const uint64_t v[8] = {ticket,           ald(&p.hash),  ald(&p.res[sk][0]),
+                         ald(&p.res[sk][1]), ald(&p.s0[0]), ald(&p.s0[1]),
                          ald(&p.t_admit),  t_done};
   uint64_t* dst = &c->ticket;  // the first eight words, in field order
   for (int i = 0; i < 8; i++)
     __hip_atomic_store(dst + i, v[i], __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_SYSTEM);
-  __hip_atomic_store(&c->iters, ald(&p.stop_iters), __ATOMIC_RELAXED,
+  __hip_atomic_store(&c->iters, iters, __ATOMIC_RELAXED,
                      __HIP_MEMORY_SCOPE_SYSTEM);
   __hip_atomic_store(&c->slot, slot, __ATOMIC_RELAXED,
                      __HIP_MEMORY_SCOPE_SYSTEM);
@@ -238,7 +247,7 @@ __device__ void admit(const EngineDesc& E, uint64_t idx, Request* s_req) {
     p.r = q;
     ast(&p.inflight, (uint32_t)(q.tr * q.tc));
     ast(&p.state, 0u);
-    ast(&p.stop_iters, 0u);
+    ast(&p.stop_iters, 0xffffffffu);  // no stop decided yet
     for (int i = 0; i < 4; i++) {
       ast(&p.iter_tiles[i], 0u);
       ast(&p.res[i][0], 0ull);
@@ -246,8 +255,6 @@ __device__ void admit(const EngineDesc& E, uint64_t idx, Request* s_req) {
     }
     ast(&p.s0[0], 0ull);
     ast(&p.s0[1], 0ull);
-    ast(&p.s_last[0], 0ull);
-    ast(&p.s_last[1], 0ull);
     ast(&p.hash, 0ull);
     ast(&p.t_admit, (uint64_t)wall_clock64());
     aadd(&E.ctl[CW_ACTIVE], 1ull);
@@ -255,11 +262,12 @@ __device__ void admit(const EngineDesc& E, uint64_t idx, Request* s_req) {
   __syncthreads();
   const Request q = uni(*s_req);
   const uint32_t n_tiles = (uint32_t)(q.tr * q.tc);
+  const uint32_t gate = q.tol2 >= 0 ? 1u : 0u;  // sweep 1 waits for S_0
   for (uint32_t u = threadIdx.x; u < n_tiles; u += blockDim.x) {
     uint32_t nb[4];
     const uint32_t deg = (uint32_t)tile_neighbors(q, u, nb);
     q.pend[3 * u + 0] = q.init_on_device ? 1 + deg : 0;  // sweep 0
-    q.pend[3 * u + 1] = 1 + deg;                         // sweep 1: no gate
+    q.pend[3 * u + 1] = 1 + deg + gate;                  // sweep 1
     q.pend[3 * u + 2] = 0;                               // sweeps done
   }
   drain();
@@ -403,13 +411,21 @@ __global__ void __launch_bounds__(T) engine_kernel(EngineDesc E) {
     __syncthreads();
     const uint32_t n_tiles = (uint32_t)(r.tr * r.tc);
     if (uni(s_flags) & F_GATE) {
-      // the decision on sweep k lets sweep k+2 (parity k & 1) go
-      for (uint32_t u = threadIdx.x; u < n_tiles; u += blockDim.x)
-        if (asub(&r.pend[3 * u + (k & 1)], 1u) == 1) {
+      // the decision on sweep k lets sweep k+2 (parity k & 1) go, and the
+      // decision on sweep 0 also sweep 1
+      const bool g1 = k == 0 && 1 < r.max_iters;
+      const bool g2 = k + 2 < r.max_iters;
+      for (uint32_t u = threadIdx.x; u < n_tiles; u += blockDim.x) {
+        const int q = (int)((blockIdx.x + u) % (uint32_t)E.n_wg);
+        if (g1 && asub(&r.pend[3 * u + 1], 1u) == 1) {
           aadd(&p.inflight, 1u);
-          push_task(E, task_word(K_ITER, slot, u),
-                    (int)((blockIdx.x + u) % (uint32_t)E.n_wg));
+          push_task(E, task_word(K_ITER, slot, u), q);
         }
+        if (g2 && asub(&r.pend[3 * u + (k & 1)], 1u) == 1) {
+          aadd(&p.inflight, 1u);
+          push_task(E, task_word(K_ITER, slot, u), q);
+        }
+      }
       drain();
     }
     __syncthreads();

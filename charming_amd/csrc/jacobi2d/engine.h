@@ -7,7 +7,8 @@
 // releases its neighbours: per tile and per iteration parity, a counter of
 // unmet dependencies (the tile's own previous sweep, each neighbour's
 // previous sweep, and - when the problem stops on a residual - the stop
-// decision two sweeps back) is decremented by every producer; whoever takes
+// decision two sweeps back, or for sweep 1 the decision on sweep 0) is
+// decremented by every producer; whoever takes
 // it to zero enqueues the task.  Halos are read in place from the
 // neighbour's cells (the zero-copy halo mode: the message carries no
 // payload).  Ready tasks go to the completing workgroup's own queue; idle
@@ -57,13 +58,13 @@ struct Prob {
   Request r;
   uint32_t inflight;      // tasks enqueued and not finished
   uint32_t state;         // 0 running, 1 stopped, 2 hashing
-  uint32_t stop_iters;    // sweeps in the final result
+  uint32_t stop_iters;    // sweeps in the final result (min over stops)
   uint32_t pad1;
   uint32_t iter_tiles[4]; // tiles that finished sweep k (slot k & 3)
   uint32_t pad0[4];
   uint64_t res[4][2];     // residual partials of sweep k (slot k & 3)
   uint64_t s0[2];         // residual of the first sweep
-  uint64_t s_last[2];     // residual of the last sweep
+  uint64_t pad2[2];
   uint64_t hash;
   uint64_t t_admit, t_done;  // device wall clock
 };

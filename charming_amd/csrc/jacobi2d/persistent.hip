@@ -17,7 +17,12 @@
 //     width by the chare-row count, jacobi2d.cu:64-65, a transposition that
 //     is harmless only for square chare grids);
 //   - the last row/column of chares takes the remainder of an uneven split;
-//   - a chare with no neighbours iterates without waiting.
+//   - a chare with no neighbours iterates without waiting;
+//   - temporal blocking (halo depth 2, the default): chares exchange halos
+//     two cells deep (four edges of two rows/columns plus the four corner
+//     cells from the diagonal neighbours) once per two sweeps, and sweep
+//     twice in one pass over their block with the intermediate grid held in
+//     registers; the arithmetic, and so every grid, is unchanged.
 #define CHARMING_PE_THREADS 1024
 #include <charming.h>
 
@@ -33,7 +38,11 @@
 #define RIGHT 1
 #define TOP 2
 #define BOTTOM 3
-#define N_NEIGHBORS 4
+#define TOP_LEFT 4
+#define TOP_RIGHT 5
+#define BOTTOM_LEFT 6
+#define BOTTOM_RIGHT 7
+#define N_NEIGHBORS 8
 
 namespace chm {
 namespace jac {
@@ -50,7 +59,23 @@ struct JacobiDesc {
   int tune_nc, tune_u;
   int overlap;            // sweep edges first, send, then the interior
   uint64_t* recv_ticks;   // per chare: ticks inside recv_ghosts (unpack)
+  int depth;              // halo depth = sweeps per exchange (1 or 2)
 };
+
+// Chare arrays always hold halos two deep: rows -1 .. bh+2 of
+// pitch_for_depth(bw, 2) doubles; `base` points at row 0.
+constexpr int kHaloRows = 2;
+__host__ __device__ inline int64_t chare_pitch(int bw) {
+  return pitch_for_depth(bw, kHaloRows);
+}
+
+// The side a message arrives on at the receiver, by the direction it was
+// sent in.
+__host__ __device__ inline int opposite_side(int k) {
+  const int opp[8] = {RIGHT, LEFT, BOTTOM, TOP, BOTTOM_RIGHT, BOTTOM_LEFT,
+                      TOP_RIGHT, TOP_LEFT};
+  return opp[k];
+}
 
 __device__ JacobiDesc g_jdesc;
 
@@ -79,8 +104,10 @@ struct Block : charm::chare {
   int block_width, block_height;
   int64_t pitch;
   int n_iters, iter;
-  double* grids[2];      // u_k lives in grids[k & 1]
-  double* boundaries[N_NEIGHBORS];  // [dir slot | edge] staging
+  int depth;             // sweeps per halo exchange
+  int cur;               // u_iter lives in grids[cur]; each round flips it
+  double* grids[2];      // (pointers to row 0 of each array)
+  double* boundaries[N_NEIGHBORS];  // [dir slot | depth x edge] staging
   // per-chare phase accounting (jacobi2d.h:31-42 keeps send/recv/update
   // times the same way): device wall-clock ticks
   uint64_t t_sweep, t_send, t_wait, t_mark;
@@ -228,6 +255,60 @@ __device__ void sweep_dispatch(const double* in, double* out, int64_t P,
     chare_sweep<1, 2, false>(in, out, P, bw, bh, eo, phase);
 }
 
+// Two fused sweeps of the whole block (halos two deep), panels x bands.
+template <int NC, int U, bool RES, bool PACK>
+__device__ __noinline__ void chare_sweep2(const double* in, double* out,
+                                          int64_t P, int bw, int bh,
+                                          EdgeOut2 eo, RingSides rs) {
+  const int wave = threadIdx.x / 64, WV = blockDim.x / 64;
+  const int pw = 128 * NC;
+  const int S = (bw + pw - 1) / pw;
+  int nb = WV / S;
+  if (nb < 1) nb = 1;
+  if (nb > bh) nb = bh;
+  const int br = (bh + nb - 1) / nb;
+  u128acc acc = {0, 0};
+  for (int t = wave; t < S * nb; t += WV) {
+    const int panel = t % S, b = t / S;
+    const int r0 = 1 + b * br, r1 = min(r0 + br, bh + 1);
+    sweep_panel2<NC, U, RES, PACK>(in, out, P, bw, bh, 1 + panel * pw, r0, r1,
+                                   acc, eo, rs);
+  }
+  if (RES) {
+    acc = wave_reduce_u128(acc);
+    if (__lane_id() == 0) atomic_add_u128(g_jdesc.res_acc, acc);
+  }
+}
+
+__device__ void sweep2_dispatch(const double* in, double* out, int64_t P,
+                                int bw, int bh, bool res, bool pack,
+                                const EdgeOut2& eo, RingSides rs) {
+  int nc = g_jdesc.tune_nc, u = g_jdesc.tune_u;
+  if (nc == 0) {
+    nc = 1;
+    u = 4;
+  }
+  if (u == 0) u = 2;
+#define CHM_CASE2(NC_, U_)                                                   \
+  if (nc == NC_ && u == U_) {                                                \
+    if (res)                                                                 \
+      chare_sweep2<NC_, U_, true, false>(in, out, P, bw, bh, eo, rs);        \
+    else if (pack)                                                           \
+      chare_sweep2<NC_, U_, false, true>(in, out, P, bw, bh, eo, rs);        \
+    else                                                                     \
+      chare_sweep2<NC_, U_, false, false>(in, out, P, bw, bh, eo, rs);       \
+    return;                                                                  \
+  }
+  CHM_CASE2(1, 2) CHM_CASE2(1, 4) CHM_CASE2(2, 2)
+#undef CHM_CASE2
+  if (res)
+    chare_sweep2<1, 4, true, false>(in, out, P, bw, bh, eo, rs);
+  else if (pack)
+    chare_sweep2<1, 4, false, true>(in, out, P, bw, bh, eo, rs);
+  else
+    chare_sweep2<1, 4, false, false>(in, out, P, bw, bh, eo, rs);
+}
+
 __device__ void Block::init(void* arg) {
   const JacobiDesc& d = g_jdesc;
   if (threadIdx.x == 0) {
@@ -238,42 +319,57 @@ __device__ void Block::init(void* arg) {
     const ChareGeom g = chare_geom(params[0], params[1], cr, cc, index);
     block_width = g.bw;
     block_height = g.bh;
-    pitch = pitch_for(g.bw);
+    pitch = chare_pitch(g.bw);
     iter = 0;
+    cur = 0;
+    depth = d.depth;
     row = index / cc;
     col = index % cc;
     neighbor_index[LEFT] = (col == 0) ? -1 : index - 1;
     neighbor_index[RIGHT] = (col == cc - 1) ? -1 : index + 1;
     neighbor_index[TOP] = (row == 0) ? -1 : index - cc;
     neighbor_index[BOTTOM] = (row == cr - 1) ? -1 : index + cc;
+    // diagonal neighbours supply the corner cells of two-deep halos
+    const bool dg = depth == 2;
+    neighbor_index[TOP_LEFT] = (dg && row > 0 && col > 0) ? index - cc - 1 : -1;
+    neighbor_index[TOP_RIGHT] =
+        (dg && row > 0 && col < cc - 1) ? index - cc + 1 : -1;
+    neighbor_index[BOTTOM_LEFT] =
+        (dg && row < cr - 1 && col > 0) ? index + cc - 1 : -1;
+    neighbor_index[BOTTOM_RIGHT] =
+        (dg && row < cr - 1 && col < cc - 1) ? index + cc + 1 : -1;
     neighbor_count = 0;
     for (int k = 0; k < N_NEIGHBORS; k++)
       neighbor_count += (neighbor_index[k] >= 0);
     recv_count = 0;
     end_count = 0;
     t_sweep = t_send = t_wait = 0;
-    grids[0] = d.arena + (int64_t)(2 * index) * d.chare_elems;
+    grids[0] = d.arena + (int64_t)(2 * index) * d.chare_elems + pitch;
     grids[1] = grids[0] + d.chare_elems;
     for (int k = 0; k < N_NEIGHBORS; k++) {
-      boundaries[k] = d.staging + (int64_t)(4 * index + k) * d.stage_elems;
+      boundaries[k] = d.staging + (int64_t)(N_NEIGHBORS * index + k) * d.stage_elems;
       // dir tag = the receiver's side (jacobi2d.cu:139-140)
-      ((int*)boundaries[k])[0] = (k % 2 == 0) ? k + 1 : k - 1;
+      ((int*)boundaries[k])[0] = opposite_side(k);
       ((int*)boundaries[k])[1] = 0;
     }
     block_proxy->set_refnum(index, 0);
   }
   __syncthreads();
-  // pack the edges of u_0 (later iterations pack inside the sweep)
+  // pack the edges of u_0, `depth` deep (later rounds pack inside the sweep)
   const double* t = grids[0];
-  const int bw = block_width, bh = block_height;
+  const int bw = block_width, bh = block_height, D = depth;
   const int64_t P = pitch;
-  for (int k = threadIdx.x; k < bw; k += blockDim.x) {
-    boundaries[TOP][1 + k] = t[P + kColOff + 1 + k];
-    boundaries[BOTTOM][1 + k] = t[P * bh + kColOff + 1 + k];
-  }
-  for (int k = threadIdx.x; k < bh; k += blockDim.x) {
-    boundaries[LEFT][1 + k] = t[P * (1 + k) + kColOff + 1];
-    boundaries[RIGHT][1 + k] = t[P * (1 + k) + kColOff + bw];
+  for (int dd = 0; dd < D; dd++) {
+    for (int k = threadIdx.x; k < bw; k += blockDim.x) {
+      boundaries[TOP][1 + dd * bw + k] = t[P * (1 + dd) + kColOff + 1 + k];
+      boundaries[BOTTOM][1 + dd * bw + k] =
+          t[P * (bh - D + 1 + dd) + kColOff + 1 + k];
+    }
+    for (int k = threadIdx.x; k < bh; k += blockDim.x) {
+      boundaries[LEFT][1 + dd * bh + k] = t[P * (1 + k) + kColOff + 1 + dd];
+      boundaries[RIGHT][1 + dd * bh + k] =
+          t[P * (1 + k) + kColOff + bw - D + 1 + dd];
+    }
   }
   __syncthreads();
   send_boundaries();
@@ -282,13 +378,27 @@ __device__ void Block::init(void* arg) {
 }
 
 __device__ void Block::send_boundaries() {
-  // (jacobi2d.cu:152-194) four sends tagged with the iteration
-  const int it = iter;
+  // (jacobi2d.cu:152-194) sends tagged with the iteration: four edges
+  // `depth` deep, and with two-deep halos the four corner cells, taken from
+  // the top and bottom edge staging (a diagonal neighbour implies both)
+  const int it = iter, bw = block_width, D = depth;
+  if (D == 2 && threadIdx.x == 0) {
+    const double* top = boundaries[TOP] + 1;
+    const double* bot = boundaries[BOTTOM] + 1 + (D - 1) * bw;
+    if (neighbor_index[TOP_LEFT] >= 0) boundaries[TOP_LEFT][1] = top[0];
+    if (neighbor_index[TOP_RIGHT] >= 0) boundaries[TOP_RIGHT][1] = top[bw - 1];
+    if (neighbor_index[BOTTOM_LEFT] >= 0) boundaries[BOTTOM_LEFT][1] = bot[0];
+    if (neighbor_index[BOTTOM_RIGHT] >= 0)
+      boundaries[BOTTOM_RIGHT][1] = bot[bw - 1];
+  }
+  __syncthreads();
   for (int k = 0; k < N_NEIGHBORS; k++) {
     if (neighbor_index[k] < 0) continue;
-    const int len = (k == LEFT || k == RIGHT) ? block_height : block_width;
+    size_t len = 1;
+    if (k == LEFT || k == RIGHT) len = (size_t)D * block_height;
+    if (k == TOP || k == BOTTOM) len = (size_t)D * block_width;
     block_proxy->invoke(neighbor_index[k], 1, boundaries[k],
-                        sizeof(double) + sizeof(double) * (size_t)len, it);
+                        sizeof(double) + sizeof(double) * len, it);
   }
 }
 
@@ -297,25 +407,40 @@ __device__ void Block::recv_ghosts(void* arg) {
   const uint64_t tr0 = wall_clock64();
   const int dir = *(const int*)arg;
   const double* ghost = (const double*)arg + 1;
-  double* t = grids[iter & 1];
+  double* t = grids[cur];
   const int bw = block_width, bh = block_height;
   const int64_t P = pitch;
+  const int D = depth;
+  // edge messages hold `depth` rows/columns, nearest the sender's far side
+  // first: they land in halo rows/columns 1-D .. 0 or n+1 .. n+D
   switch (dir) {
     case LEFT:
-      for (int k = threadIdx.x; k < bh; k += blockDim.x)
-        t[P * (1 + k) + kColOff] = ghost[k];
+      for (int k = threadIdx.x; k < D * bh; k += blockDim.x)
+        t[P * (1 + k % bh) + kColOff + 1 - D + k / bh] = ghost[k];
       break;
     case RIGHT:
-      for (int k = threadIdx.x; k < bh; k += blockDim.x)
-        t[P * (1 + k) + kColOff + bw + 1] = ghost[k];
+      for (int k = threadIdx.x; k < D * bh; k += blockDim.x)
+        t[P * (1 + k % bh) + kColOff + bw + 1 + k / bh] = ghost[k];
       break;
     case TOP:
-      for (int k = threadIdx.x; k < bw; k += blockDim.x)
-        t[kColOff + 1 + k] = ghost[k];
+      for (int k = threadIdx.x; k < D * bw; k += blockDim.x)
+        t[P * (1 - D + k / bw) + kColOff + 1 + k % bw] = ghost[k];
       break;
     case BOTTOM:
-      for (int k = threadIdx.x; k < bw; k += blockDim.x)
-        t[P * (bh + 1) + kColOff + 1 + k] = ghost[k];
+      for (int k = threadIdx.x; k < D * bw; k += blockDim.x)
+        t[P * (bh + 1 + k / bw) + kColOff + 1 + k % bw] = ghost[k];
+      break;
+    case TOP_LEFT:
+      if (threadIdx.x == 0) t[kColOff] = ghost[0];
+      break;
+    case TOP_RIGHT:
+      if (threadIdx.x == 0) t[kColOff + bw + 1] = ghost[0];
+      break;
+    case BOTTOM_LEFT:
+      if (threadIdx.x == 0) t[P * (bh + 1) + kColOff] = ghost[0];
+      break;
+    case BOTTOM_RIGHT:
+      if (threadIdx.x == 0) t[P * (bh + 1) + kColOff + bw + 1] = ghost[0];
       break;
     default:
       if (threadIdx.x == 0) charm::rt::raise_error(charm::rt::ERR_BAD_MESSAGE, dir);
@@ -342,9 +467,12 @@ __device__ void Block::update() {
       t0 = wall_clock64();
       t_wait += t0 - t_mark;
     }
-    const bool last = (iter == n_iters - 1);
-    double* t = grids[iter & 1];
-    double* nt = grids[(iter + 1) & 1];
+    // two fused sweeps per exchange with two-deep halos (a lone final
+    // sweep when the count is odd), else one
+    const int steps = (depth == 2 && n_iters - iter >= 2) ? 2 : 1;
+    const bool last = (iter + steps == n_iters);
+    double* t = grids[cur];
+    double* nt = grids[cur ^ 1];
     EdgeOut eo;
     eo.top = (neighbor_index[TOP] >= 0) ? boundaries[TOP] + 1 : nullptr;
     eo.bottom = (neighbor_index[BOTTOM] >= 0) ? boundaries[BOTTOM] + 1 : nullptr;
@@ -352,18 +480,26 @@ __device__ void Block::update() {
     eo.right = (neighbor_index[RIGHT] >= 0) ? boundaries[RIGHT] + 1 : nullptr;
     // overlap: phase 0 = everything that feeds the outgoing halos, sent
     // before phase 1 sweeps the interior; otherwise one whole-block sweep
-    const bool ov = g_jdesc.overlap != 0;
-    if (ov)
+    const bool ov = g_jdesc.overlap != 0 && depth == 1;
+    if (steps == 2) {
+      EdgeOut2 eo2 = {eo.top, eo.bottom, eo.left, eo.right};
+      const RingSides rs = {neighbor_index[TOP] < 0, neighbor_index[BOTTOM] < 0,
+                            neighbor_index[LEFT] < 0,
+                            neighbor_index[RIGHT] < 0};
+      sweep2_dispatch(t, nt, P, bw, bh, last, !last, eo2, rs);
+    } else if (ov) {
       sweep_dispatch(t, nt, P, bw, bh, last, eo, 0);
-    else
+    } else {
       sweep_dispatch(t, nt, P, bw, bh, last, eo, 2);
+    }
     __syncthreads();
     __shared__ int done;
     uint64_t t1 = 0, t2 = 0;
     if (threadIdx.x == 0) {
       t1 = wall_clock64();
       t_sweep += t1 - t0;
-      ++iter;
+      iter += steps;
+      cur ^= 1;
       done = (iter == n_iters);
       if (!done) block_proxy->set_refnum(charm::chare::i, iter);
     }
@@ -413,27 +549,30 @@ namespace jac {
 
 // u_0 for every chare: interior from the counter-based generator at global
 // coordinates (decomposition invariant), halo ring = ring value, padding 0.
+// Halo cells get u_0 of the global cell they mirror (ring value on the
+// ring), so the values there are right before the first exchange; cells
+// outside the global grid and padding get 0.
 __global__ void init_chares_kernel(JacobiDesc d, uint64_t seed, int mode,
                                    double boundary) {
   const int c = blockIdx.y;
   const ChareGeom g = chare_geom(d.W, d.H, d.cr, d.cc, c);
-  const int64_t P = pitch_for(g.bw);
-  const int64_t n = (int64_t)(g.bh + 2) * P;
+  const int64_t P = chare_pitch(g.bw);
+  const int64_t n = (int64_t)(g.bh + 2 * kHaloRows) * P;
   double* a = d.arena + (int64_t)(2 * c) * d.chare_elems;
   double* b = a + d.chare_elems;
   const double ring = (mode == CHM_INIT_REFERENCE) ? 10.0 : boundary;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = e / P, j = e % P - kColOff;
+    const int64_t i = e / P - (kHaloRows - 1), j = e % P - kColOff;
+    const int64_t gi = g.row0 + i, gj = g.col0 + j;
     double v = 0.0;
-    if (j >= 0 && j <= g.bw + 1) {
-      const bool interior = i >= 1 && i <= g.bh && j >= 1 && j <= g.bw;
+    if (j >= 1 - kHaloRows && j <= g.bw + kHaloRows && gi >= 0 &&
+        gi <= d.H + 1 && gj >= 0 && gj <= d.W + 1) {
+      const bool interior = gi >= 1 && gi <= d.H && gj >= 1 && gj <= d.W;
       if (!interior)
         v = ring;
       else
-        v = (mode == CHM_INIT_REFERENCE)
-                ? 0.0
-                : cell_u01(seed, g.row0 + i, g.col0 + j);
+        v = (mode == CHM_INIT_REFERENCE) ? 0.0 : cell_u01(seed, gi, gj);
     }
     a[e] = v;
     b[e] = v;
@@ -445,8 +584,8 @@ __global__ void hash_chares_kernel(JacobiDesc d, int parity, int r0, int r1,
                                    int c0, int c1, uint64_t* out) {
   const int c = blockIdx.y;
   const ChareGeom g = chare_geom(d.W, d.H, d.cr, d.cc, c);
-  const int64_t P = pitch_for(g.bw);
-  const double* a = d.arena + (int64_t)(2 * c + parity) * d.chare_elems;
+  const int64_t P = chare_pitch(g.bw);
+  const double* a = d.arena + (int64_t)(2 * c + parity) * d.chare_elems + P;
   const int lr0 = max(r0, g.row0 + 1), lr1 = min(r1, g.row0 + 1 + g.bh);
   const int lc0 = max(c0, g.col0 + 1), lc1 = min(c1, g.col0 + 1 + g.bw);
   uint64_t h = 0;
@@ -468,6 +607,7 @@ struct PersistentState {
   JacobiDesc desc;
   ChareGeom max_geom;
   int n_chares;
+  int min_side;  // shortest chare side
   bool own_runtime;
 };
 
@@ -537,13 +677,17 @@ int persistent_create(Handle* h) {
   // the last chare row/column is the largest
   const ChareGeom gmax = chare_geom(h->W, h->H, cr, cc, cr * cc - 1);
   ps->max_geom = gmax;
-  d.chare_elems = (int64_t)array_elems(gmax.bw, gmax.bh);
+  d.chare_elems = (int64_t)array_elems_depth(gmax.bw, gmax.bh, kHaloRows);
   d.chare_elems = (d.chare_elems + 31) / 32 * 32;  // 256-byte aligned slots
-  d.stage_elems = ((int64_t)std::max(gmax.bw, gmax.bh) + 2 + 15) / 16 * 16;
+  d.stage_elems =
+      ((int64_t)kHaloRows * std::max(gmax.bw, gmax.bh) + 2 + 15) / 16 * 16;
+  // the smallest chare decides whether two-deep halos are possible
+  const ChareGeom gmin = chare_geom(h->W, h->H, cr, cc, 0);
+  ps->min_side = std::min(gmin.bw, gmin.bh);
   const size_t arena_bytes =
       (size_t)ps->n_chares * 2 * (size_t)d.chare_elems * sizeof(double);
-  const size_t stage_bytes =
-      (size_t)ps->n_chares * 4 * (size_t)d.stage_elems * sizeof(double);
+  const size_t stage_bytes = (size_t)ps->n_chares * N_NEIGHBORS *
+                             (size_t)d.stage_elems * sizeof(double);
   if (hipMalloc(&d.arena, arena_bytes) != hipSuccess ||
       hipMalloc(&d.staging, stage_bytes) != hipSuccess ||
       hipMalloc(&d.res_acc, 4 * sizeof(uint64_t)) != hipSuccess ||
@@ -576,6 +720,12 @@ int persistent_solve(Handle* h, chm_jacobi_result* res) {
   d.tune_nc = h->tune.panel_chunks;
   d.tune_u = h->tune.rows_in_flight;
   d.overlap = h->tune.overlap;
+  // two-deep halos need every chare at least 2 x 2
+  d.depth = h->tune.depth ? h->tune.depth : (ps->min_side >= 2 ? 2 : 1);
+  if (d.depth != 1 && d.depth != 2)
+    return fail(CHM_E_INVALID, "halo depth %d (1 or 2)", d.depth);
+  if (d.depth == 2 && ps->min_side < 2)
+    return fail(CHM_E_INVALID, "halo depth 2 needs chares of at least 2x2");
   CHM_TRY(upload_desc(d));
   CHM_TRY_HIP(hipMemset(d.res_acc, 0, 4 * sizeof(uint64_t)));
   CHM_TRY_HIP(hipMemset(d.recv_ticks, 0, (size_t)ps->n_chares * 8));
@@ -585,7 +735,9 @@ int persistent_solve(Handle* h, chm_jacobi_result* res) {
   } else {
     const int rc = chm_run();
     if (rc != CHM_OK) return fail(rc, "persistent run: %s", chm_last_error());
-    h->cur = d.iters & 1;
+    // one array flip per exchange round
+    const int rounds = d.depth == 2 ? (d.iters + 1) / 2 : d.iters;
+    h->cur = rounds & 1;
   }
   if (res) {
     memset(res, 0, sizeof(*res));
@@ -628,8 +780,8 @@ int persistent_read_grid(Handle* h, double* out) {
   for (int64_t e = 0; e < GP * (h->H + 2); e++) out[e] = ring;
   for (int c = 0; c < ps->n_chares; c++) {
     const ChareGeom g = chare_geom(d.W, d.H, d.cr, d.cc, c);
-    const int64_t P = pitch_for(g.bw);
-    const double* a = d.arena + (int64_t)(2 * c + h->cur) * d.chare_elems;
+    const int64_t P = chare_pitch(g.bw);
+    const double* a = d.arena + (int64_t)(2 * c + h->cur) * d.chare_elems + P;
     CHM_TRY_HIP(hipMemcpy2DAsync(
         out + (int64_t)(g.row0 + 1) * GP + g.col0 + 1, GP * sizeof(double),
         a + P + kColOff + 1, P * sizeof(double), (size_t)g.bw * sizeof(double),

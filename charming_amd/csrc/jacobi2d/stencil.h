@@ -37,6 +37,15 @@ __host__ __device__ inline int64_t pitch_for(int w) {
 __host__ __device__ inline size_t array_elems(int w, int h) {
   return (size_t)(h + 2) * (size_t)pitch_for(w);
 }
+// Arrays with halos `d` cells deep (temporal blocking over d sweeps): rows
+// 1-d .. h+d, columns 1-d .. w+d at element offset 15 + col; the pitch keeps
+// two spare elements after column w+d for the pair loads.
+__host__ __device__ inline int64_t pitch_for_depth(int w, int d) {
+  return ((int64_t)w + 16 + 2 * d + 15) / 16 * 16;
+}
+__host__ __device__ inline size_t array_elems_depth(int w, int h, int d) {
+  return (size_t)(h + 2 * d) * (size_t)pitch_for_depth(w, d);
+}
 
 // ----------------------------------------------------------------------------
 // exact fixed-point residual: sum floor((a-b)^2 * 2^56) in 128 bits
@@ -280,6 +289,215 @@ __device__ inline void sweep_panel(const double* __restrict__ in,
         p.template compute<RES, PACK>(R[s], R[(s + 1) % K], R[(s + 2) % K],
                                       r + s, acc, eo);
         if (r + s + U + 1 <= r1) p.load(R[s], r + s + U + 1);
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Two sweeps fused (temporal blocking, depth-2 halos)
+// ----------------------------------------------------------------------------
+// u_{k+1} is computed in registers on one extra row and column on every side
+// of the output region and never stored; u_{k+2} is computed from it with
+// the same expression, so both are bit-identical to two separate sweeps.
+// HBM traffic per two sweeps: one read of u_k and one write of u_{k+2}.
+
+// Outgoing halos two deep (chare form): top = rows 1,2 (w each), bottom =
+// rows h-1,h, left = columns 1,2 (h each), right = columns w-1,w.
+struct EdgeOut2 {
+  double* top;
+  double* bottom;
+  double* left;
+  double* right;
+};
+
+// Which sides of the array are the fixed Dirichlet ring (not a neighbour's
+// halo): there u_{k+1} is the ring value, not a stencil result.
+struct RingSides {
+  bool top, bottom, left, right;
+};
+
+template <int NC>
+struct Row2 {      // a row of u_k
+  double2 v[NC];
+  double e1, e2;   // lane 0: columns c0-1, c0-2; lane 63: c0+128NC, c0+128NC+1
+};
+template <int NC>
+struct MidRow {    // a row of u_{k+1}
+  double2 v[NC];
+  double e;        // lane 0: column c0-1; lane 63: column c0+128NC
+};
+
+template <int NC>
+struct Panel2 {
+  const gcdouble* in;
+  gdouble* out;
+  int64_t pitch;
+  int w, h, c0, cend, lane;
+  int col1, col2;  // columns this lane loads into e1 / e2 (-1000: none)
+  RingSides ring;
+
+  __device__ inline void setup(const double* in_, double* out_, int64_t p,
+                               int w_, int h_, int c0_, int cend_,
+                               RingSides rs) {
+    in = (const gcdouble*)in_;
+    out = (gdouble*)out_;
+    pitch = p;
+    w = w_;
+    h = h_;
+    c0 = c0_;
+    cend = cend_;
+    ring = rs;
+    lane = __lane_id();
+    col1 = col2 = -1000;
+    if (lane == 0) {
+      col1 = c0 - 1;
+      col2 = c0 - 2;
+    }
+    if (lane == 63) {
+      if (c0 + 128 * NC <= w + 2) col1 = c0 + 128 * NC;
+      if (c0 + 128 * NC + 1 <= w + 2) col2 = c0 + 128 * NC + 1;
+    }
+  }
+  __device__ inline int col(int k) const { return c0 + 128 * k + 2 * lane; }
+
+  __device__ inline void load(Row2<NC>& r, int row) const {
+    const gcdouble* rp = in + (int64_t)row * pitch + kColOff;
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      const int c = col(k);
+      if (c <= w + 2)
+        r.v[k] = gload2(rp + c);
+      else
+        r.v[k] = make_double2(0.0, 0.0);
+    }
+    r.e1 = (col1 > -1000) ? rp[col1] : 0.0;
+    r.e2 = (col2 > -1000) ? rp[col2] : 0.0;
+  }
+
+  // u_{k+1} of row q over the panel's columns and one more on each side
+  __device__ inline void mid(const Row2<NC>& up, const Row2<NC>& cu,
+                             const Row2<NC>& dn, int q, MidRow<NC>& m) const {
+    if ((q == 0 && ring.top) || (q == h + 1 && ring.bottom)) {
+#pragma unroll
+      for (int k = 0; k < NC; k++) m.v[k] = cu.v[k];
+      m.e = cu.e1;
+      return;
+    }
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      const double a = cu.v[k].x, b = cu.v[k].y;
+      double bl = from_left_lane(b);
+      double ar = from_right_lane(a);
+      if (lane == 0) bl = (k == 0) ? cu.e1 : readlane_d(cu.v[k - 1].y, 63);
+      if (lane == 63)
+        ar = (k == NC - 1) ? cu.e1 : readlane_d(cu.v[(k + 1) % NC].x, 0);
+      // c + W + E + N + S, then * 0.2 (jacobi2d.cu:332-333)
+      double na = (a + bl + b + up.v[k].x + dn.v[k].x) * 0.2;
+      double nb = (b + a + ar + up.v[k].y + dn.v[k].y) * 0.2;
+      const int c = col(k);
+      if (ring.right) {
+        if (c == w + 1) na = a;
+        if (c + 1 == w + 1) nb = b;
+      }
+      m.v[k] = make_double2(na, nb);
+    }
+    // the edge columns: lane 0 holds c0-1 (west c0-2, east c0), lane 63
+    // holds c0+128NC (west c0+128NC-1, east c0+128NC+1)
+    const double ew = (lane == 0) ? cu.e2 : cu.v[NC - 1].y;
+    const double ee = (lane == 0) ? cu.v[0].x : cu.e2;
+    double me = (cu.e1 + ew + ee + up.e1 + dn.e1) * 0.2;
+    if (lane == 0 && ring.left && c0 == 1) me = cu.e1;
+    if (lane == 63 && ring.right && c0 + 128 * NC == w + 1) me = cu.e1;
+    m.e = me;
+  }
+
+  // u_{k+2} of row `row` from three rows of u_{k+1}; stores, residual
+  // against u_{k+1}, and the two-deep outgoing edges
+  template <bool RES, bool PACK>
+  __device__ inline void outrow(const MidRow<NC>& up, const MidRow<NC>& cu,
+                                const MidRow<NC>& dn, int row, u128acc& acc,
+                                const EdgeOut2& eo) const {
+    gdouble* orow = out + (int64_t)row * pitch + kColOff;
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      const double a = cu.v[k].x, b = cu.v[k].y;
+      double bl = from_left_lane(b);
+      double ar = from_right_lane(a);
+      if (lane == 0) bl = (k == 0) ? cu.e : readlane_d(cu.v[k - 1].y, 63);
+      if (lane == 63)
+        ar = (k == NC - 1) ? cu.e : readlane_d(cu.v[(k + 1) % NC].x, 0);
+      const double na = (a + bl + b + up.v[k].x + dn.v[k].x) * 0.2;
+      const double nb = (b + a + ar + up.v[k].y + dn.v[k].y) * 0.2;
+      const int c = col(k);
+      const bool two = c + 1 <= cend, one = c == cend;
+      if (two) {
+        gstore2(orow + c, na, nb);
+        if (RES) {
+          acc_add(acc, res_term(na, a));
+          acc_add(acc, res_term(nb, b));
+        }
+      } else if (one) {
+        orow[c] = na;
+        if (RES) acc_add(acc, res_term(na, a));
+      }
+      if (PACK && (two || one)) {
+        if (row <= 2 && eo.top) {
+          eo.top[(int64_t)(row - 1) * w + c - 1] = na;
+          if (two) eo.top[(int64_t)(row - 1) * w + c] = nb;
+        }
+        if (row >= h - 1 && eo.bottom) {
+          eo.bottom[(int64_t)(row - (h - 1)) * w + c - 1] = na;
+          if (two) eo.bottom[(int64_t)(row - (h - 1)) * w + c] = nb;
+        }
+        if (eo.left) {
+          if (c <= 2) eo.left[(int64_t)(c - 1) * h + row - 1] = na;
+          if (two && c + 1 <= 2) eo.left[(int64_t)c * h + row - 1] = nb;
+        }
+        if (eo.right) {
+          if (c >= w - 1) eo.right[(int64_t)(c - (w - 1)) * h + row - 1] = na;
+          if (two && c + 1 >= w - 1)
+            eo.right[(int64_t)(c + 1 - (w - 1)) * h + row - 1] = nb;
+        }
+      }
+    }
+  }
+};
+
+// Two fused sweeps of rows [r0, r1) of the panel at column c0: reads u_k
+// rows r0-2 .. r1+1 (columns c0-2 .. c0+128NC+1), writes u_{k+2} rows
+// r0 .. r1-1, columns c0 .. min(c0+128NC-1, cend).  u_k rows sit in a ring
+// of U+2 register slots (U rows in flight, static indices by unrolling);
+// the three live rows of u_{k+1} shift through registers.
+template <int NC, int U, bool RES, bool PACK>
+__device__ inline void sweep_panel2(const double* __restrict__ in,
+                                    double* __restrict__ out, int64_t pitch,
+                                    int w, int h, int c0, int r0, int r1,
+                                    u128acc& acc, const EdgeOut2& eo,
+                                    RingSides rs, int cend = -1) {
+  if (r0 >= r1) return;
+  constexpr int K = U + 2;
+  Panel2<NC> p;
+  p.setup(in, out, pitch, w, h, c0, cend < 0 ? w : cend, rs);
+  Row2<NC> R[K];
+  MidRow<NC> m0, m1, m2;
+#pragma unroll
+  for (int s = 0; s < K; s++)
+    if (r0 - 2 + s <= r1 + 1) p.load(R[s], r0 - 2 + s);
+  // step t computes u_{k+1} row q = r0-1+t from u_k rows q-1 (slot t % K),
+  // q, q+1; then emits u_{k+2} row q-1 once three u_{k+1} rows exist
+  const int n_steps = r1 - r0 + 2;
+  for (int t0 = 0; t0 < n_steps; t0 += K) {
+#pragma unroll
+    for (int s = 0; s < K; s++) {
+      const int t = t0 + s;
+      if (t < n_steps) {
+        const int q = r0 - 1 + t;
+        m0 = m1;
+        m1 = m2;
+        p.mid(R[s], R[(s + 1) % K], R[(s + 2) % K], q, m2);
+        if (q - 1 + K <= r1 + 1) p.load(R[s], q - 1 + K);
+        if (t >= 2) p.template outrow<RES, PACK>(m0, m1, m2, q - 1, acc, eo);
       }
     }
   }

@@ -66,7 +66,9 @@ typedef struct chm_jacobi_tuning {
   int band_rows;     /* rows per wave band (plain form) */
   int wg_waves;      /* waves per workgroup (plain form) */
   int overlap;       /* persistent form: 1 = sweep the edges first and send the
-                        halos before the interior (default 0) */
+                        halos before the interior (default 0; depth 1 only) */
+  int depth;         /* persistent form: halo depth = sweeps per exchange,
+                        1 or 2 (0 = 2 when every chare is at least 2x2) */
 } chm_jacobi_tuning;
 
 /* Allocate the grids of one problem in HBM for the given form and fill u_0.

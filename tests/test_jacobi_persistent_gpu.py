@@ -16,6 +16,13 @@ def cam():
     return charming_amd
 
 
+def expected_messages(cr, cc, iters, depth):
+    edges = 2 * (cr * (cc - 1) + cc * (cr - 1))
+    corners = 4 * (cr - 1) * (cc - 1) if depth == 2 else 0
+    rounds = (iters + 1) // 2 if depth == 2 else iters
+    return 2 * cr * cc + rounds * (edges + corners)
+
+
 def test_c1_config_matches_chare_oracle(cam):
     # 512^2 interior, 4x4 chares of 128^2, 100 iterations: full grid + residual
     W = H = 512
@@ -27,7 +34,9 @@ def test_c1_config_matches_chare_oracle(cam):
     assert np.array_equal(g, ref)
     assert r.residual == res
     assert r.hash == O.grid_hash(ref)
-    assert r.messages >= 48 * 100  # halos (+ init and end messages)
+    # init + end messages, and per exchange round 48 edges and 36 corners
+    # (two-deep halos: one round per two sweeps)
+    assert r.messages == expected_messages(4, 4, 100, 2)
 
 
 @pytest.mark.parametrize("W,H,cr,cc,iters", [
@@ -77,3 +86,41 @@ def test_default_decomposition_uses_all_pes(cam):
     ref, res = O.jacobi_run(W, H, iters, seed=9)
     assert h_full == O.grid_hash(ref)
     assert r.residual == res
+
+
+@pytest.mark.parametrize("depth", [1, 2])
+@pytest.mark.parametrize("W,H,cr,cc,iters", [
+    (300, 200, 3, 5, 17), (257, 129, 4, 3, 10), (64, 64, 2, 2, 1),
+    (100, 100, 5, 5, 2), (130, 70, 1, 1, 9), (500, 40, 2, 7, 5)])
+def test_halo_depths_bit_exact(cam, depth, W, H, cr, cc, iters):
+    ref, res = O.jacobi_run(W, H, iters, seed=W + H)
+    with cam.Jacobi2D(W, H, iters, form="persistent", seed=W + H,
+                      chares=(cr, cc)) as j:
+        j.set_tuning(depth=depth)
+        r = j.solve()
+        assert np.array_equal(j.grid(), ref)
+        assert r.residual == res and r.hash == O.grid_hash(ref)
+        assert r.messages == expected_messages(cr, cc, iters, depth)
+
+
+@pytest.mark.parametrize("nc,u", [(1, 2), (1, 4), (2, 2)])
+def test_fused_sweep_shapes(cam, nc, u):
+    W, H, iters = 1000, 600, 12
+    ref, res = O.jacobi_run(W, H, iters, seed=21, mode=1)
+    with cam.Jacobi2D(W, H, iters, form="persistent", seed=21, chares=(3, 2),
+                      init_mode=cam.INIT_REFERENCE) as j:
+        j.set_tuning(nc, u, depth=2)
+        r = j.solve()
+        assert np.array_equal(j.grid(), ref) and r.residual == res
+
+
+def test_thin_chares_fall_back_to_depth_one(cam):
+    # one-row chares: the automatic depth is 1, and depth 2 is refused
+    ref, res = O.jacobi_run(64, 8, 6, seed=5)
+    with cam.Jacobi2D(64, 8, 6, form="persistent", seed=5, chares=(8, 2)) as j:
+        r = j.solve()
+        assert np.array_equal(j.grid(), ref) and r.residual == res
+        j.reset()
+        j.set_tuning(depth=2)
+        with pytest.raises(cam.ChmError):
+            j.solve()
