@@ -81,9 +81,10 @@ def test_result_alone_equals_in_mix(cam):
 
 
 def test_admission_against_budget(cam):
-    # two 1024^2 requests need ~34 MB of grids each; a 48 MiB budget holds one
+    # a 1024^2 request needs two arrays of 1026 x 1056 doubles (17.3 MB);
+    # a 24 MiB budget holds one
     p = {"width": 1024, "height": 1024, "iters": 200, "tol": 1e-9, "seed": 3}
-    with cam.Server("engine", budget_bytes=48 << 20) as srv:
+    with cam.Server("engine", budget_bytes=24 << 20) as srv:
         t1 = srv.submit(p)
         assert t1 is not None
         assert srv.submit(dict(p, seed=4)) is None  # BUSY: budget in use
@@ -117,8 +118,7 @@ def test_continuous_batching_overlaps_requests(cam):
             time.sleep(0.0005)
     assert order[-1] == tl  # the short ones did not wait for the long one
     for t, i in shorts.items():
-        h, res = O.grid_hash(O.jacobi_run(64, 64, 20, seed=50 + i)[0]), None
-        assert got[t]["hash"] == h
+        assert got[t]["hash"] == O.grid_hash(O.jacobi_run(64, 64, 20, seed=50 + i)[0])
 
 
 def test_submit_rejects_bad_request(cam):
