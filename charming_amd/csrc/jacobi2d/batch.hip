@@ -39,57 +39,97 @@ using jac::kColOff;
 // task bodies (workgroup-collective)
 // ---------------------------------------------------------------------------
 
-// Seed a tile's cells, and the ring cells beside it, in both arrays.
+// u_0 value of array cell (i, j): seeded interior, ring, 0 outside the grid.
+__device__ inline double u0_cell(const Request& r, int64_t i, int64_t j) {
+  if (i < 0 || i > r.H + 1 || j < 0 || j > r.W + 1) return 0.0;
+  const bool interior = i >= 1 && i <= r.H && j >= 1 && j <= r.W;
+  if (!interior) return (r.init_mode == CHM_INIT_REFERENCE) ? 10.0 : r.boundary;
+  return (r.init_mode == CHM_INIT_REFERENCE) ? 0.0 : jac::cell_u01(r.seed, i, j);
+}
+
+// Seed a tile's cells, and at the grid's edge the two halo rings beside it,
+// in both arrays.
 __device__ void do_init(const Request& r, const TileGeom& g) {
-  const int i0 = g.r0 - (g.r0 == 1 ? 1 : 0), i1 = g.r1 + (g.r1 == r.H + 1 ? 1 : 0);
-  const int j0 = g.c0 - (g.c0 == 1 ? 1 : 0), j1 = g.c1 + (g.c1 == r.W + 1 ? 1 : 0);
+  const int i0 = g.r0 - (g.r0 == 1 ? 2 : 0), i1 = g.r1 + (g.r1 == r.H + 1 ? 2 : 0);
+  const int j0 = g.c0 - (g.c0 == 1 ? 2 : 0), j1 = g.c1 + (g.c1 == r.W + 1 ? 2 : 0);
   const int64_t cols = j1 - j0, n = (int64_t)(i1 - i0) * cols;
-  const double ring = (r.init_mode == CHM_INIT_REFERENCE) ? 10.0 : r.boundary;
   for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
     const int64_t i = i0 + e / cols, j = j0 + e % cols;
-    const bool interior = i >= 1 && i <= r.H && j >= 1 && j <= r.W;
-    double v = ring;
-    if (interior)
-      v = (r.init_mode == CHM_INIT_REFERENCE) ? 0.0 : jac::cell_u01(r.seed, i, j);
+    const double v = u0_cell(r, i, j);
     const int64_t o = i * r.pitch + kColOff + j;
     r.a[o] = v;
     r.b[o] = v;
   }
 }
 
-// One sweep of a tile: the tile's columns are cut into panels of 128*NC and
-// its rows into bands so every wave of the workgroup gets one panel-band.
-template <int NC, int U, bool RES>
-__device__ void do_sweep(const Request& r, const TileGeom& g, int k,
-                         uint64_t* res_slot) {
-  const double* in = (k & 1) ? r.b : r.a;
-  double* out = (k & 1) ? r.a : r.b;
-  const int nW = blockDim.x / 64, wave = threadIdx.x / 64;
-  const int tw = g.c1 - g.c0, th = g.r1 - g.r0;
-  const int pw = 128 * NC;
-  const int S = (tw + pw - 1) / pw;
-  int nb = nW / S;
-  if (nb < 1) nb = 1;
-  if (nb > th) nb = th;
-  const int br = (th + nb - 1) / nb;
-  jac::u128acc acc = {0, 0};
-  const EdgeOut eo = {nullptr, nullptr, nullptr, nullptr};
-  for (int t = wave; t < S * nb; t += nW) {
-    const int panel = t % S, b = t / S;
-    const int rr0 = g.r0 + b * br, rr1 = min(rr0 + br, g.r1);
-    jac::sweep_panel<NC, U, RES, false>(in, out, r.pitch, r.W, r.H,
-                                        g.c0 + panel * pw, rr0, rr1, acc, eo,
-                                        g.c1 - 1);
-  }
-  if (RES) {
-    acc = jac::wave_reduce_u128(acc);
-    if (__lane_id() == 0 && (acc.lo | acc.hi)) jac::atomic_add_u128(res_slot, acc);
-  }
+// Waves of the workgroup split a tile into panels of 128*NC columns x bands.
+struct TileSplit {
+  int S, nb, br, pw;
+};
+template <int NC>
+__device__ inline TileSplit split_tile(const TileGeom& g) {
+  TileSplit t;
+  const int nW = blockDim.x / 64;
+  t.pw = 128 * NC;
+  t.S = (g.c1 - g.c0 + t.pw - 1) / t.pw;
+  t.nb = nW / t.S;
+  if (t.nb < 1) t.nb = 1;
+  if (t.nb > g.r1 - g.r0) t.nb = g.r1 - g.r0;
+  t.br = (g.r1 - g.r0 + t.nb - 1) / t.nb;
+  return t;
 }
 
-// Hash of the tile's interior cells of the result (u_iters).
-__device__ uint64_t do_hash(const Request& r, const TileGeom& g, int iters) {
-  const double* u = (iters & 1) ? r.b : r.a;
+__device__ inline void add_residual(uint64_t* slot, jac::u128acc acc) {
+  acc = jac::wave_reduce_u128(acc);
+  if (__lane_id() == 0 && (acc.lo | acc.hi)) jac::atomic_add_u128(slot, acc);
+}
+
+// One sweep of a tile from array `par` into the other.
+template <int NC, int U, bool RES>
+__device__ void do_sweep(const Request& r, const TileGeom& g, int par,
+                         uint64_t* res_slot) {
+  const double* in = par ? r.b : r.a;
+  double* out = par ? r.a : r.b;
+  const TileSplit ts = split_tile<NC>(g);
+  const int nW = blockDim.x / 64, wave = threadIdx.x / 64;
+  jac::u128acc acc = {0, 0};
+  const EdgeOut eo = {nullptr, nullptr, nullptr, nullptr};
+  for (int t = wave; t < ts.S * ts.nb; t += nW) {
+    const int panel = t % ts.S, b = t / ts.S;
+    const int rr0 = g.r0 + b * ts.br, rr1 = min(rr0 + ts.br, g.r1);
+    jac::sweep_panel<NC, U, RES, false>(in, out, r.pitch, r.W, r.H,
+                                        g.c0 + panel * ts.pw, rr0, rr1, acc,
+                                        eo, g.c1 - 1);
+  }
+  if (RES) add_residual(res_slot, acc);
+}
+
+// Two fused sweeps of a tile from array `par` into the other; RESA / RESB:
+// residuals of the first / second sweep into res_slot[0..1] / [2..3].
+template <int NC, int U, bool RESA, bool RESB>
+__device__ void do_round2(const Request& r, const TileGeom& g, int par,
+                          uint64_t* res_slot) {
+  const double* in = par ? r.b : r.a;
+  double* out = par ? r.a : r.b;
+  const TileSplit ts = split_tile<NC>(g);
+  const int nW = blockDim.x / 64, wave = threadIdx.x / 64;
+  jac::u128acc acc_a = {0, 0}, acc_b = {0, 0};
+  const jac::EdgeOut2 eo = {nullptr, nullptr, nullptr, nullptr};
+  const jac::RingSides rs = {true, true, true, true};  // the grid's ring
+  for (int t = wave; t < ts.S * ts.nb; t += nW) {
+    const int panel = t % ts.S, b = t / ts.S;
+    const int rr0 = g.r0 + b * ts.br, rr1 = min(rr0 + ts.br, g.r1);
+    jac::sweep_panel2<NC, U, RESA, RESB, false>(
+        in, out, r.pitch, r.W, r.H, g.c0 + panel * ts.pw, rr0, rr1, acc_a,
+        acc_b, eo, rs, g.c1 - 1);
+  }
+  if (RESA) add_residual(res_slot, acc_a);
+  if (RESB) add_residual(res_slot + 2, acc_b);
+}
+
+// Hash of the tile's interior cells of array `par`.
+__device__ uint64_t do_hash(const Request& r, const TileGeom& g, int par) {
+  const double* u = par ? r.b : r.a;
   const int64_t cols = g.c1 - g.c0, n = (int64_t)(g.r1 - g.r0) * cols;
   uint64_t h = 0;
   for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
@@ -103,17 +143,17 @@ __device__ uint64_t do_hash(const Request& r, const TileGeom& g, int iters) {
 // ---------------------------------------------------------------------------
 // completion logic
 // ---------------------------------------------------------------------------
-enum : int { F_GATE = 1, F_FINALIZE = 2, F_PUBLISH = 4 };
+enum : int { F_GATE = 1, F_FINALIZE = 2, F_PUBLISH = 4, F_FIX = 8 };
 
 __host__ __device__ inline double u128d(uint64_t lo, uint64_t hi) {
   return ldexp((double)hi, 64) + (double)lo;
 }
 
 // Thread 0: decrement the counters (word `word`) of `tile` and its
-// neighbours; queue the sweep tasks this makes ready.
+// neighbours; queue the round tasks this makes ready.
 __device__ void release_deps(const EngineDesc& E, Prob& p, const Request& r,
                              uint32_t slot, uint32_t tile, int word) {
-  uint32_t nb[4], ready[5];
+  uint32_t nb[8], ready[9];
   const int deg = tile_neighbors(r, tile, nb);
   int n = 0;
   if (asub(&r.pend[3 * tile + word], 1u) == 1) ready[n++] = tile;
@@ -124,65 +164,89 @@ __device__ void release_deps(const EngineDesc& E, Prob& p, const Request& r,
   for (int i = 0; i < n; i++) push_task(E, task_word(K_ITER, slot, ready[i]));
 }
 
-// Thread 0, after sweep k of `tile`: release sweep k+1 of the tile and its
-// neighbours; the last tile of a sweep whose residual matters decides
-// whether the problem stops.  Returns F_GATE when the decision lets later
-// sweeps of every tile go (their gates are then dropped by the whole
-// workgroup).
+// Thread 0, after round rr of `tile`: release round rr+1 of the tile and its
+// neighbours; the last tile of a round whose residual matters decides
+// whether the problem stops.  Returns F_GATE when the decision lets round
+// rr+1 of every tile go (its gate is then dropped by the whole workgroup).
 //
-// Decisions on sweeps k and k+1 can finish in either order (sweep k+1 only
-// waits for the decision on k-1), so a stop keeps the smallest sweep count
-// (atomic min) and leaves that sweep's residual slot untouched; the result
-// u_{k+1} is intact because sweep k+1 writes the other array and sweep k+2
-// is never released.
-__device__ int complete_iter(const EngineDesc& E, Prob& p, const Request& r,
-                             uint32_t slot, uint32_t tile, int k) {
-  if (k + 1 < r.max_iters) release_deps(E, p, r, slot, tile, (k + 1) & 1);
+// With a tolerance every round waits for the decision on the previous one,
+// so decisions are made in order and the arrays of round rr are intact when
+// it is decided: a stop after the first sweep of a fused round (whose result
+// was not stored) is repaired by redoing that sweep alone (K_FIX tasks).
+// The rule is the monolithic tolerance solve's: stop after the first sweep
+// k with S_k <= tol^2 * S_0, in the same doubles.
+__device__ int complete_round(const EngineDesc& E, Prob& p, const Request& r,
+                              uint32_t slot, uint32_t tile, int rr) {
+  if (rr + 1 < r.rounds) release_deps(E, p, r, slot, tile, (rr + 1) & 1);
   const bool tol = r.tol2 >= 0;
-  const bool last = k + 1 == r.max_iters;
+  const bool last = rr + 1 == r.rounds;
   if (!tol && !last) return 0;
   const uint32_t n_tiles = (uint32_t)(r.tr * r.tc);
-  const int sk = tol ? (k & 3) : 0;
-  if (aadd(&p.iter_tiles[sk], 1u) != n_tiles - 1) return 0;
-  // every tile has finished sweep k: its residual is complete
-  const uint64_t lo = ald(&p.res[sk][0]), hi = ald(&p.res[sk][1]);
-  uint64_t s0lo, s0hi;
-  if (k == 0) {
-    s0lo = lo;
-    s0hi = hi;
-    ast(&p.s0[0], lo);
-    ast(&p.s0[1], hi);
-  } else {  // sweep 1 waited for the decision on sweep 0
-    s0lo = ald(&p.s0[0]);
-    s0hi = ald(&p.s0[1]);
+  const int sk = tol ? (rr & 1) : 0;
+  if (aadd(&p.round_tiles[sk], 1u) != n_tiles - 1) return 0;
+  // every tile has finished round rr: its residuals are complete
+  const int n_sw = min(2, r.max_iters - 2 * rr);
+  const uint64_t alo = ald(&p.res[sk][0][0]), ahi = ald(&p.res[sk][0][1]);
+  const uint64_t blo = ald(&p.res[sk][1][0]), bhi = ald(&p.res[sk][1][1]);
+  int stop = 0;  // sweeps in the result when stopping
+  uint64_t slo = 0, shi = 0;
+  if (!tol) {
+    stop = r.max_iters;
+    slo = n_sw == 2 ? blo : alo;
+    shi = n_sw == 2 ? bhi : ahi;
+  } else {
+    uint64_t s0lo, s0hi;
+    if (rr == 0) {
+      s0lo = alo;
+      s0hi = ahi;
+      ast(&p.s0[0], alo);
+      ast(&p.s0[1], ahi);
+    } else {
+      s0lo = ald(&p.s0[0]);
+      s0hi = ald(&p.s0[1]);
+    }
+    const double thr = r.tol2 * u128d(s0lo, s0hi);
+    if (u128d(alo, ahi) <= thr || (last && n_sw == 1)) {
+      stop = 2 * rr + 1;
+      slo = alo;
+      shi = ahi;
+      if (n_sw == 2) ast(&p.fix_round, (uint32_t)rr);
+    } else if (n_sw == 2 && (u128d(blo, bhi) <= thr || last)) {
+      stop = 2 * rr + 2;
+      slo = blo;
+      shi = bhi;
+    }
   }
-  // the stop rule of the monolithic tolerance solve, in the same doubles
-  const bool stop = last || u128d(lo, hi) <= r.tol2 * u128d(s0lo, s0hi);
   if (stop) {
-    __hip_atomic_fetch_min(&p.stop_iters, (uint32_t)(k + 1), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+    ast(&p.s_last[0], slo);
+    ast(&p.s_last[1], shi);
+    ast(&p.stop_iters, (uint32_t)stop);
     ast(&p.state, 1u);
   } else {
-    ast(&p.iter_tiles[sk], 0u);
-    ast(&p.res[sk][0], 0ull);
-    ast(&p.res[sk][1], 0ull);
+    ast(&p.round_tiles[sk], 0u);
+    for (int i = 0; i < 2; i++) {
+      ast(&p.res[sk][i][0], 0ull);
+      ast(&p.res[sk][i][1], 0ull);
+    }
   }
   release_agent();
-  return (!stop && tol) ? F_GATE : 0;
+  return stop ? 0 : F_GATE;
 }
 
 // Thread 0: a task of the problem is finished.  The last task of a stopped
-// problem starts the hash tasks (F_FINALIZE); the last hash task publishes
+// problem starts the repair sweep (F_FIX) or the hash tasks (F_FINALIZE);
+// the last repair task starts the hash tasks; the last hash task publishes
 // the completion (F_PUBLISH).
 __device__ int finish_task(Prob& p, uint32_t n_tiles) {
   if (asub(&p.inflight, 1u) != 1) return 0;
   acquire_agent();
   const uint32_t st = ald(&p.state);
-  if (st == 1) {
-    ast(&p.state, 2u);
+  if (st == 1 || st == 4) {
+    const bool fix = st == 1 && ald(&p.fix_round) != 0xffffffffu;
+    ast(&p.state, fix ? 4u : 2u);
     aadd(&p.inflight, n_tiles);
     release_agent();
-    return F_FINALIZE;
+    return fix ? F_FIX : F_FINALIZE;
   }
   return st == 2 ? F_PUBLISH : 0;
 }
@@ -194,16 +258,14 @@ __device__ void publish(const EngineDesc& E, Prob& p, uint32_t slot) {
   const uint64_t idx = aadd(&E.ctl[CW_DONE], 1ull);
   Completion* c = &E.dones[idx & (E.req_cap - 1)];
   const uint64_t ticket = p.r.ticket;
-  const uint32_t iters = ald(&p.stop_iters);
-  const int sk = p.r.tol2 >= 0 ? (int)((iters - 1) & 3) : 0;
-  const uint64_t v[8] = {ticket,           ald(&p.hash),  ald(&p.res[sk][0]),
-                         ald(&p.res[sk][1]), ald(&p.s0[0]), ald(&p.s0[1]),
-                         ald(&p.t_admit),  t_done};
+  const uint64_t v[8] = {ticket,             ald(&p.hash),  ald(&p.s_last[0]),
+                         ald(&p.s_last[1]),  ald(&p.s0[0]), ald(&p.s0[1]),
+                         ald(&p.t_admit),    t_done};
   uint64_t* dst = &c->ticket;  // the first eight words, in field order
   for (int i = 0; i < 8; i++)
     __hip_atomic_store(dst + i, v[i], __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_SYSTEM);
-  __hip_atomic_store(&c->iters, iters, __ATOMIC_RELAXED,
+  __hip_atomic_store(&c->iters, ald(&p.stop_iters), __ATOMIC_RELAXED,
                      __HIP_MEMORY_SCOPE_SYSTEM);
   __hip_atomic_store(&c->slot, slot, __ATOMIC_RELAXED,
                      __HIP_MEMORY_SCOPE_SYSTEM);
@@ -241,20 +303,25 @@ __device__ void admit(const EngineDesc& E, uint64_t idx, Request* s_req) {
     q.seed = vr->seed;
     q.ticket = vr->ticket;
     q.slot = vr->slot;
-    q.pad = 0;
+    q.rounds = vr->rounds;
     *s_req = q;
     Prob& p = E.probs[q.slot & (kMaxSlots - 1)];
     p.r = q;
     ast(&p.inflight, (uint32_t)(q.tr * q.tc));
     ast(&p.state, 0u);
-    ast(&p.stop_iters, 0xffffffffu);  // no stop decided yet
-    for (int i = 0; i < 4; i++) {
-      ast(&p.iter_tiles[i], 0u);
-      ast(&p.res[i][0], 0ull);
-      ast(&p.res[i][1], 0ull);
+    ast(&p.stop_iters, 0u);
+    ast(&p.fix_round, 0xffffffffu);
+    for (int i = 0; i < 2; i++) {
+      ast(&p.round_tiles[i], 0u);
+      for (int j = 0; j < 2; j++) {
+        ast(&p.res[i][j][0], 0ull);
+        ast(&p.res[i][j][1], 0ull);
+      }
     }
     ast(&p.s0[0], 0ull);
     ast(&p.s0[1], 0ull);
+    ast(&p.s_last[0], 0ull);
+    ast(&p.s_last[1], 0ull);
     ast(&p.hash, 0ull);
     ast(&p.t_admit, (uint64_t)wall_clock64());
     aadd(&E.ctl[CW_ACTIVE], 1ull);
@@ -262,13 +329,13 @@ __device__ void admit(const EngineDesc& E, uint64_t idx, Request* s_req) {
   __syncthreads();
   const Request q = uni(*s_req);
   const uint32_t n_tiles = (uint32_t)(q.tr * q.tc);
-  const uint32_t gate = q.tol2 >= 0 ? 1u : 0u;  // sweep 1 waits for S_0
+  const uint32_t gate = q.tol2 >= 0 ? 1u : 0u;  // round 1 waits for decision 0
   for (uint32_t u = threadIdx.x; u < n_tiles; u += blockDim.x) {
-    uint32_t nb[4];
+    uint32_t nb[8];
     const uint32_t deg = (uint32_t)tile_neighbors(q, u, nb);
-    q.pend[3 * u + 0] = q.init_on_device ? 1 + deg : 0;  // sweep 0
-    q.pend[3 * u + 1] = 1 + deg + gate;                  // sweep 1
-    q.pend[3 * u + 2] = 0;                               // sweeps done
+    q.pend[3 * u + 0] = q.init_on_device ? 1 + deg : 0;  // round 0
+    q.pend[3 * u + 1] = 1 + deg + gate;                  // round 1
+    q.pend[3 * u + 2] = 0;                               // rounds done
   }
   drain();
   __syncthreads();
@@ -332,6 +399,30 @@ __device__ int find_work(const EngineDesc& E, uint32_t* task, uint64_t* req,
   }
 }
 
+// Round rr of a tile: two fused sweeps (or the last single one), with the
+// residuals the problem needs.
+template <int NC, int U>
+__device__ void run_round(const Request& r, const TileGeom& g, int rr,
+                          Prob& p) {
+  const bool tol = r.tol2 >= 0;
+  const bool last = rr + 1 == r.rounds;
+  const int n_sw = min(2, r.max_iters - 2 * rr);
+  uint64_t* slot = &p.res[tol ? (rr & 1) : 0][0][0];
+  const int par = rr & 1;
+  if (n_sw == 1) {
+    if (tol || last)
+      do_sweep<NC, U, true>(r, g, par, slot);
+    else
+      do_sweep<NC, U, false>(r, g, par, slot);
+  } else if (tol) {
+    do_round2<NC, U, true, true>(r, g, par, slot);
+  } else if (last) {
+    do_round2<NC, U, false, true>(r, g, par, slot);
+  } else {
+    do_round2<NC, U, false, false>(r, g, par, slot);
+  }
+}
+
 template <int T, int NC, int U>
 __global__ void __launch_bounds__(T) engine_kernel(EngineDesc E) {
   __shared__ uint32_t s_task;
@@ -363,14 +454,16 @@ __global__ void __launch_bounds__(T) engine_kernel(EngineDesc E) {
       int k = 0;
       if (kind == K_ITER) {
         k = (int)ald(&s_r.pend[3 * tile + 2]);
-        // re-arm this parity's counter for sweep k+2: the tile's and its
-        // neighbours' sweep k+1, and the stop decision of sweep k
-        uint32_t nb[4];
+        // re-arm this parity's counter for round k+2: the tile's and its
+        // neighbours' round k+1, and the stop decision on round k+1
+        uint32_t nb[8];
         const uint32_t deg = (uint32_t)tile_neighbors(s_r, tile, nb);
         const uint32_t gate = s_r.tol2 >= 0 ? 1u : 0u;
         ast(&s_r.pend[3 * tile + (k & 1)], 1u + deg + gate);
       } else if (kind == K_HASH) {
         k = (int)ald(&p.stop_iters);
+      } else if (kind == K_FIX) {
+        k = (int)ald(&p.fix_round);
       }
       s_k = k;
     }
@@ -381,15 +474,12 @@ __global__ void __launch_bounds__(T) engine_kernel(EngineDesc E) {
     if (kind == K_INIT) {
       do_init(r, g);
     } else if (kind == K_ITER) {
-      const bool tol = r.tol2 >= 0;
-      const bool res = tol || k == r.max_iters - 1;
-      uint64_t* slot_res = &p.res[tol ? (k & 3) : 0][0];
-      if (res)
-        do_sweep<NC, U, true>(r, g, k, slot_res);
-      else
-        do_sweep<NC, U, false>(r, g, k, slot_res);
+      run_round<NC, U>(r, g, k, p);
+    } else if (kind == K_FIX) {
+      do_sweep<NC, U, false>(r, g, k & 1, nullptr);
     } else {
-      const uint64_t h = do_hash(r, g, k);
+      // the result of `k` sweeps is in array ceil(k / 2) & 1
+      const uint64_t h = do_hash(r, g, ((k + 1) / 2) & 1);
       if (__lane_id() == 0 && h) aadd(&p.hash, h);
     }
     drain();
@@ -401,9 +491,10 @@ __global__ void __launch_bounds__(T) engine_kernel(EngineDesc E) {
       if (kind == K_INIT) {
         release_deps(E, p, r, slot, tile, 0);
       } else if (kind == K_ITER) {
-        f = complete_iter(E, p, r, slot, tile, k);
+        f = complete_round(E, p, r, slot, tile, k);
+        const uint64_t sweeps = (uint64_t)min(2, r.max_iters - 2 * k);
         aadd(&E.ctl[CW_STATS + 3],
-             (uint64_t)(g.r1 - g.r0) * (uint64_t)(g.c1 - g.c0));
+             sweeps * (uint64_t)(g.r1 - g.r0) * (uint64_t)(g.c1 - g.c0));
       }
       aadd(&E.ctl[CW_STATS + 0], 1ull);
       s_flags = f;
@@ -411,20 +502,14 @@ __global__ void __launch_bounds__(T) engine_kernel(EngineDesc E) {
     __syncthreads();
     const uint32_t n_tiles = (uint32_t)(r.tr * r.tc);
     if (uni(s_flags) & F_GATE) {
-      // the decision on sweep k lets sweep k+2 (parity k & 1) go, and the
-      // decision on sweep 0 also sweep 1
-      const bool g1 = k == 0 && 1 < r.max_iters;
-      const bool g2 = k + 2 < r.max_iters;
-      for (uint32_t u = threadIdx.x; u < n_tiles; u += blockDim.x) {
-        const int q = (int)((blockIdx.x + u) % (uint32_t)E.n_wg);
-        if (g1 && asub(&r.pend[3 * u + 1], 1u) == 1) {
-          aadd(&p.inflight, 1u);
-          push_task(E, task_word(K_ITER, slot, u), q);
-        }
-        if (g2 && asub(&r.pend[3 * u + (k & 1)], 1u) == 1) {
-          aadd(&p.inflight, 1u);
-          push_task(E, task_word(K_ITER, slot, u), q);
-        }
+      // the decision on round k lets round k+1 (parity (k+1) & 1) go
+      if (k + 1 < r.rounds) {
+        for (uint32_t u = threadIdx.x; u < n_tiles; u += blockDim.x)
+          if (asub(&r.pend[3 * u + ((k + 1) & 1)], 1u) == 1) {
+            aadd(&p.inflight, 1u);
+            push_task(E, task_word(K_ITER, slot, u),
+                      (int)((blockIdx.x + u) % (uint32_t)E.n_wg));
+          }
       }
       drain();
     }
@@ -432,9 +517,10 @@ __global__ void __launch_bounds__(T) engine_kernel(EngineDesc E) {
     if (threadIdx.x == 0) s_flags = finish_task(p, n_tiles);
     __syncthreads();
     const int f = uni(s_flags);
-    if (f & F_FINALIZE) {
+    if (f & (F_FINALIZE | F_FIX)) {
+      const uint32_t kk = (f & F_FIX) ? K_FIX : K_HASH;
       for (uint32_t u = threadIdx.x; u < n_tiles; u += blockDim.x)
-        push_task(E, task_word(K_HASH, slot, u),
+        push_task(E, task_word(kk, slot, u),
                   (int)((blockIdx.x + u) % (uint32_t)E.n_wg));
     } else if ((f & F_PUBLISH) && threadIdx.x == 0) {
       publish(E, p, slot);
@@ -450,23 +536,16 @@ __global__ void init_queues(uint64_t* cells, uint64_t n_cells, uint32_t qcap) {
     cells[i] = (uint64_t)(i & (qcap - 1)) << 32;
 }
 
-// Host-launched seeding of a whole problem: u_0 in both arrays, padding 0.
+// Host-launched seeding of a whole problem: u_0 in both arrays (rows -1 ..
+// H+2), padding 0.
 __global__ void init_problem(Request r) {
-  const int64_t n = (int64_t)(r.H + 2) * r.pitch;
-  const double ring = (r.init_mode == CHM_INIT_REFERENCE) ? 10.0 : r.boundary;
+  const int64_t n = (int64_t)(r.H + 4) * r.pitch;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = e / r.pitch, j = e % r.pitch - kColOff;
-    double v = 0.0;
-    if (j >= 0 && j <= r.W + 1) {
-      const bool interior = i >= 1 && i <= r.H && j >= 1 && j <= r.W;
-      v = interior ? ((r.init_mode == CHM_INIT_REFERENCE)
-                          ? 0.0
-                          : jac::cell_u01(r.seed, i, j))
-                   : ring;
-    }
-    r.a[e] = v;
-    r.b[e] = v;
+    const int64_t i = e / r.pitch - 1, j = e % r.pitch - kColOff;
+    const double v = (j >= -1 && j <= r.W + 2) ? u0_cell(r, i, j) : 0.0;
+    r.a[e - r.pitch] = v;
+    r.b[e - r.pitch] = v;
   }
 }
 
@@ -557,8 +636,10 @@ static Built build_request(const chm_batch_problem& pb, int block) {
   r.boundary = pb.boundary;
   r.tol2 = pb.tol > 0 ? pb.tol * pb.tol : -1.0;
   r.seed = pb.seed;
-  r.pitch = jac::pitch_for(r.W);
-  b.grid_bytes = round_up(jac::array_elems(r.W, r.H) * sizeof(double), 256);
+  r.rounds = (pb.iters + 1) / 2;
+  r.pitch = jac::pitch_for_depth(r.W, 2);  // halos two deep: fused rounds
+  b.grid_bytes =
+      round_up(jac::array_elems_depth(r.W, r.H, 2) * sizeof(double), 256);
   b.pend_bytes = round_up((size_t)r.tr * r.tc * 3 * sizeof(uint32_t), 256);
   return b;
 }
@@ -641,6 +722,18 @@ static int check_shape(int nc, int u) {
               nc, u);
 }
 
+typedef void (*engine_fn_t)(EngineDesc);
+static engine_fn_t pick_engine(int threads, int nc, int u) {
+#define CHM_PICK(T_, NC_, U_)                         \
+  if (threads == T_ && nc == NC_ && u == U_)          \
+    return &engine_kernel<T_, NC_, U_>;
+#define CHM_PICK3(T_) CHM_PICK(T_, 1, 2) CHM_PICK(T_, 1, 4) CHM_PICK(T_, 2, 2)
+  CHM_PICK3(256) CHM_PICK3(512) CHM_PICK3(1024)
+#undef CHM_PICK3
+#undef CHM_PICK
+  return nullptr;
+}
+
 // Queues, rings and control words.  qtasks bounds the tasks that can be
 // queued at once (every tile of every problem in flight).  serving: the
 // request ring and its tail live in pinned host memory.
@@ -659,7 +752,13 @@ static int engine_setup(Engine& e, int threads, int nc, int u, uint32_t req_cap,
   e.threads = threads;
   EngineDesc& d = e.d;
   memset(&d, 0, sizeof(d));
-  d.n_wg = prop.multiProcessorCount * (1024 / threads);
+  // as many workgroups as stay resident together (registers permitting)
+  const engine_fn_t fn = pick_engine(threads, nc, u);
+  int per_cu = 0;
+  CHM_TRY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, reinterpret_cast<const void*>(fn), threads, 0));
+  if (per_cu < 1) return fail(CHM_E_HIP, "engine kernel does not fit a CU");
+  d.n_wg = prop.multiProcessorCount * std::min(per_cu, 1024 / threads);
   d.n_pollers = std::min(d.n_wg, 16);
   d.nc = nc;
   d.u = u;
@@ -704,18 +803,6 @@ static int engine_setup(Engine& e, int threads, int nc, int u, uint32_t req_cap,
   CHM_TRY_HIP(hipEventCreate(&e.ev1));
   CHM_TRY_HIP(hipDeviceSynchronize());
   return CHM_OK;
-}
-
-typedef void (*engine_fn_t)(EngineDesc);
-static engine_fn_t pick_engine(int threads, int nc, int u) {
-#define CHM_PICK(T_, NC_, U_)                         \
-  if (threads == T_ && nc == NC_ && u == U_)          \
-    return &engine_kernel<T_, NC_, U_>;
-#define CHM_PICK3(T_) CHM_PICK(T_, 1, 2) CHM_PICK(T_, 1, 4) CHM_PICK(T_, 2, 2)
-  CHM_PICK3(256) CHM_PICK3(512) CHM_PICK3(1024)
-#undef CHM_PICK3
-#undef CHM_PICK
-  return nullptr;
 }
 
 static int engine_launch(Engine& e) {
@@ -792,9 +879,9 @@ static void place_batch(std::vector<Built>& bl, char* arena) {
   size_t off = 0;
   for (size_t i = 0; i < bl.size(); i++) {
     Request& r = bl[i].r;
-    r.a = (double*)(arena + off);
+    r.a = (double*)(arena + off) + r.pitch;  // row 0 (row -1 comes first)
     off += bl[i].grid_bytes;
-    r.b = (double*)(arena + off);
+    r.b = (double*)(arena + off) + r.pitch;
     off += bl[i].grid_bytes;
     r.pend = (uint32_t*)(arena + off);
     off += bl[i].pend_bytes;
@@ -1387,8 +1474,8 @@ int chm_jacobi2d_server_submit(void* handle, const chm_batch_problem* pb,
     return fail(CHM_E_BUSY, "budget: %zu of %zu bytes in use", s->used,
                 s->pool_size);
   Request& r = b.r;
-  r.a = (double*)(s->pool + off);
-  r.b = (double*)(s->pool + off + b.grid_bytes);
+  r.a = (double*)(s->pool + off) + r.pitch;  // row 0 (row -1 comes first)
+  r.b = (double*)(s->pool + off + b.grid_bytes) + r.pitch;
   r.pend = engine ? (uint32_t*)(s->pool + off + 2 * b.grid_bytes) : nullptr;
   r.init_on_device = 1;
   r.ticket = s->next_ticket++;

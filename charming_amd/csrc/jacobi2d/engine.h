@@ -3,16 +3,17 @@
 //
 // Many independent Jacobi2D problems share one persistent launch.  Each
 // problem is cut into square chare tiles (side chosen per problem); one task
-// is one sweep of one tile.  A tile's completion is the message that
-// releases its neighbours: per tile and per iteration parity, a counter of
-// unmet dependencies (the tile's own previous sweep, each neighbour's
-// previous sweep, and - when the problem stops on a residual - the stop
-// decision two sweeps back, or for sweep 1 the decision on sweep 0) is
-// decremented by every producer; whoever takes
-// it to zero enqueues the task.  Halos are read in place from the
-// neighbour's cells (the zero-copy halo mode: the message carries no
-// payload).  Ready tasks go to the completing workgroup's own queue; idle
-// workgroups steal from the others.
+// is one round of one tile: two sweeps fused in one pass (temporal
+// blocking), or one sweep when a single one is left.  A tile's completion is
+// the message that releases its neighbours: per tile and per round parity, a
+// counter of unmet dependencies (the tile's own previous round, each of its
+// up to eight neighbours' previous round - a fused round reads two cells
+// beyond the tile, corners included - and, when the problem stops on a
+// residual, the stop decision on the previous round) is decremented by every
+// producer; whoever takes it to zero enqueues the task.  Halos are read in
+// place from the neighbours' cells (the zero-copy halo mode: the message
+// carries no payload).  Ready tasks go to the completing workgroup's own
+// queue; idle workgroups steal from the others.
 #pragma once
 #include <stdint.h>
 
@@ -21,7 +22,10 @@ namespace eng {
 
 constexpr uint32_t kMaxSlots = 4096;        // problems in flight
 constexpr uint32_t kMaxTiles = 1u << 18;    // tiles per problem
-enum : uint32_t { K_INIT = 0, K_ITER = 1, K_HASH = 2 };
+// task kinds: seed a tile, one round of a tile, hash a tile of the result,
+// redo the first sweep of the stop round alone (a stop after an odd sweep
+// count, whose result the fused round did not store)
+enum : uint32_t { K_INIT = 0, K_ITER = 1, K_HASH = 2, K_FIX = 3 };
 
 __host__ __device__ inline uint32_t task_word(uint32_t kind, uint32_t slot,
                                               uint32_t tile) {
@@ -35,12 +39,14 @@ __host__ __device__ inline uint32_t task_tile(uint32_t w) {
   return w & (kMaxTiles - 1);
 }
 
-// A request as the host publishes it.
+// A request as the host publishes it.  Arrays have halos two deep (rows
+// -1 .. H+2); a and b point at row 0.  Round r reads array r & 1 (a = 0,
+// b = 1) and writes the other.
 struct Request {
-  double* a;          // u_k for even k
-  double* b;          // u_k for odd k
+  double* a;
+  double* b;
   uint32_t* pend;     // 3 words per tile: dependency counters of the next
-                      // even and odd sweep, sweeps done
+                      // even and odd round, rounds done
   int64_t pitch;
   int W, H, bs, tr, tc, max_iters;
   int init_on_device; // 1: the engine seeds the grids (init tasks)
@@ -50,23 +56,23 @@ struct Request {
   uint64_t seed;
   uint64_t ticket;
   uint32_t slot;
-  uint32_t pad;
+  int rounds;         // ceil(max_iters / 2)
 };
 
 // Device state of a problem slot (HBM, written only by the engine).
 struct Prob {
   Request r;
   uint32_t inflight;      // tasks enqueued and not finished
-  uint32_t state;         // 0 running, 1 stopped, 2 hashing
-  uint32_t stop_iters;    // sweeps in the final result (min over stops)
-  uint32_t pad1;
-  uint32_t iter_tiles[4]; // tiles that finished sweep k (slot k & 3)
-  uint32_t pad0[4];
-  uint64_t res[4][2];     // residual partials of sweep k (slot k & 3)
+  uint32_t state;         // 0 running, 1 stopped, 4 redoing a sweep, 2 hashing
+  uint32_t stop_iters;    // sweeps in the final result
+  uint32_t fix_round;     // round whose first sweep is redone (~0u: none)
+  uint32_t round_tiles[2];  // tiles that finished round r (slot r & 1)
+  uint32_t pad0[2];
+  uint64_t res[2][2][2];  // [r & 1][sweep 2r, 2r+1][lo, hi] residual sums
   uint64_t s0[2];         // residual of the first sweep
-  uint64_t pad2[2];
+  uint64_t s_last[2];     // residual of the last sweep of the result
   uint64_t hash;
-  uint64_t t_admit, t_done;  // device wall clock
+  uint64_t t_admit;       // device wall clock
 };
 
 // A completion record (host-coherent memory); `valid` = ticket + 1 is

@@ -92,7 +92,7 @@ __device__ inline Request uni(const Request& s) {
   r.seed = uni(s.seed);
   r.ticket = uni(s.ticket);
   r.slot = uni(s.slot);
-  r.pad = 0;
+  r.rounds = uni(s.rounds);
   return r;
 }
 
@@ -168,14 +168,18 @@ __device__ inline TileGeom tile_geom(const Request& r, uint32_t t) {
   g.c1 = min(g.c0 + r.bs, r.W + 1);
   return g;
 }
+// The up to eight tiles a fused round of tile t reads from (sides and
+// corners); returns the count.
 __device__ inline int tile_neighbors(const Request& r, uint32_t t,
                                      uint32_t* nb) {
   const int ti = (int)t / r.tc, tj = (int)t % r.tc;
   int n = 0;
-  if (tj > 0) nb[n++] = t - 1;
-  if (tj < r.tc - 1) nb[n++] = t + 1;
-  if (ti > 0) nb[n++] = t - r.tc;
-  if (ti < r.tr - 1) nb[n++] = t + r.tc;
+  for (int di = -1; di <= 1; di++)
+    for (int dj = -1; dj <= 1; dj++) {
+      const int i = ti + di, j = tj + dj;
+      if ((di | dj) == 0 || i < 0 || i >= r.tr || j < 0 || j >= r.tc) continue;
+      nb[n++] = (uint32_t)(i * r.tc + j);
+    }
   return n;
 }
 

@@ -267,12 +267,12 @@ __device__ __noinline__ void chare_sweep2(const double* in, double* out,
   if (nb < 1) nb = 1;
   if (nb > bh) nb = bh;
   const int br = (bh + nb - 1) / nb;
-  u128acc acc = {0, 0};
+  u128acc acc = {0, 0}, unused = {0, 0};
   for (int t = wave; t < S * nb; t += WV) {
     const int panel = t % S, b = t / S;
     const int r0 = 1 + b * br, r1 = min(r0 + br, bh + 1);
-    sweep_panel2<NC, U, RES, PACK>(in, out, P, bw, bh, 1 + panel * pw, r0, r1,
-                                   acc, eo, rs);
+    sweep_panel2<NC, U, false, RES, PACK>(in, out, P, bw, bh, 1 + panel * pw,
+                                          r0, r1, unused, acc, eo, rs);
   }
   if (RES) {
     acc = wave_reduce_u128(acc);

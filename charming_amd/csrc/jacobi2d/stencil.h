@@ -375,9 +375,13 @@ struct Panel2 {
     r.e2 = (col2 > -1000) ? rp[col2] : 0.0;
   }
 
-  // u_{k+1} of row q over the panel's columns and one more on each side
+  // u_{k+1} of row q over the panel's columns and one more on each side;
+  // RESA: add its residual against u_k over the output region (row q in
+  // [r0, r1) when `inside`, columns c0 .. cend)
+  template <bool RESA>
   __device__ inline void mid(const Row2<NC>& up, const Row2<NC>& cu,
-                             const Row2<NC>& dn, int q, MidRow<NC>& m) const {
+                             const Row2<NC>& dn, int q, MidRow<NC>& m,
+                             bool inside, u128acc& acc_a) const {
     if ((q == 0 && ring.top) || (q == h + 1 && ring.bottom)) {
 #pragma unroll
       for (int k = 0; k < NC; k++) m.v[k] = cu.v[k];
@@ -401,6 +405,10 @@ struct Panel2 {
         if (c + 1 == w + 1) nb = b;
       }
       m.v[k] = make_double2(na, nb);
+      if (RESA && inside) {
+        if (c <= cend) acc_add(acc_a, res_term(na, a));
+        if (c + 1 <= cend) acc_add(acc_a, res_term(nb, b));
+      }
     }
     // the edge columns: lane 0 holds c0-1 (west c0-2, east c0), lane 63
     // holds c0+128NC (west c0+128NC-1, east c0+128NC+1)
@@ -468,13 +476,16 @@ struct Panel2 {
 // rows r0-2 .. r1+1 (columns c0-2 .. c0+128NC+1), writes u_{k+2} rows
 // r0 .. r1-1, columns c0 .. min(c0+128NC-1, cend).  u_k rows sit in a ring
 // of U+2 register slots (U rows in flight, static indices by unrolling);
-// the three live rows of u_{k+1} shift through registers.
-template <int NC, int U, bool RES, bool PACK>
+// the three live rows of u_{k+1} shift through registers.  RESA / RES add
+// the exact residuals of the first / second sweep over the output region
+// to acc_a / acc.
+template <int NC, int U, bool RESA, bool RES, bool PACK>
 __device__ inline void sweep_panel2(const double* __restrict__ in,
                                     double* __restrict__ out, int64_t pitch,
                                     int w, int h, int c0, int r0, int r1,
-                                    u128acc& acc, const EdgeOut2& eo,
-                                    RingSides rs, int cend = -1) {
+                                    u128acc& acc_a, u128acc& acc,
+                                    const EdgeOut2& eo, RingSides rs,
+                                    int cend = -1) {
   if (r0 >= r1) return;
   constexpr int K = U + 2;
   Panel2<NC> p;
@@ -495,7 +506,8 @@ __device__ inline void sweep_panel2(const double* __restrict__ in,
         const int q = r0 - 1 + t;
         m0 = m1;
         m1 = m2;
-        p.mid(R[s], R[(s + 1) % K], R[(s + 2) % K], q, m2);
+        p.template mid<RESA>(R[s], R[(s + 1) % K], R[(s + 2) % K], q, m2,
+                             q >= r0 && q < r1, acc_a);
         if (q - 1 + K <= r1 + 1) p.load(R[s], q - 1 + K);
         if (t >= 2) p.template outrow<RES, PACK>(m0, m1, m2, q - 1, acc, eo);
       }

@@ -120,9 +120,10 @@ int chm_jacobi2d_abi_version(void);
  * Batched and served forms.  No reference counterpart: the reference runs
  * one Jacobi2D problem per job (examples/jacobi2d/jacobi2d.cu:26-77).  Here
  * many independent problems share one persistent launch of a task engine:
- * each problem is cut into square chare tiles, one task is one sweep of one
- * tile, a tile's completion is the (zero-copy) halo message that releases
- * its neighbours, and idle workgroups steal ready tasks from the others.
+ * each problem is cut into square chare tiles, one task is one round (two
+ * fused sweeps) of one tile, a tile's completion is the (zero-copy) halo
+ * message that releases its neighbours, and idle workgroups steal ready
+ * tasks from the others.
  * Every problem's result is bit-identical to the same problem solved alone
  * (and to the monolithic CPU sweep).
  * ------------------------------------------------------------------------ */
@@ -140,8 +141,8 @@ typedef struct chm_batch_problem {
 } chm_batch_problem;
 
 typedef struct chm_batch_config {
-  int wg_threads;     /* 256, 512 or 1024 (0 = 1024): workgroups per CU =
-                         1024 / wg_threads */
+  int wg_threads;     /* 256, 512 or 1024 (0 = 1024): up to 1024 / wg_threads
+                         workgroups per CU, as many as stay resident */
   int block;          /* tile side for problems with block 0 (0 = by size) */
   int panel_chunks;   /* sweep shape: 1 or 2 (0 = 1) */
   int rows_in_flight; /* sweep shape: 2 or 4 (0 = 2) */

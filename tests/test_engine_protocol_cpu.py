@@ -1,0 +1,279 @@
+"""CPU model of the task engine's dependency protocol (the batched and served
+forms), run under random interleavings and checked against the oracle.
+
+The model follows the device code of charming_amd/csrc/jacobi2d/batch.hip
+step for step - per-tile counters for the next even/odd round, re-arming at
+task start, up to eight neighbours, the stop-decision gate, the repair of a
+stop after the first sweep of a fused round, the in-flight count and the
+state machine - while the stencil work is done with numpy on the shared
+arrays.  Every task is split into a read phase and a write phase with other
+tasks free to run in between, so two tasks the protocol lets overlap do
+overlap.  If the protocol ever let a tile read cells a neighbour had not
+finished, or overwrite cells a neighbour still needed, the result would
+differ from the monolithic oracle's.
+"""
+import random
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+TWO56 = float(2 ** 56)
+
+
+def res_sum(new, old):
+    """Exact sum of floor((new-old)^2 * 2^56), saturating like res_term."""
+    d = new - old
+    q = d * d * TWO56
+    out = 0
+    for v in q.ravel():
+        out += (2 ** 64 - 1) if not v < 2.0 ** 64 else int(v)
+    return out
+
+
+def u128d(v):
+    return O.u128_to_double(v)
+
+
+def sweep_region(src, r0, r1, c0, c1, H, W):
+    """One sweep of rows [r0, r1) x cols [c0, c1) of src (the stencil of
+    jacobi2d.cu:332-333); cells on the grid's ring keep their value."""
+    out = src[r0:r1, c0:c1].copy()
+    i0, i1 = max(r0, 1), min(r1, H + 1)
+    j0, j1 = max(c0, 1), min(c1, W + 1)
+    if i0 < i1 and j0 < j1:
+        c = src[i0:i1, j0:j1]
+        v = (c + src[i0:i1, j0 - 1:j1 - 1] + src[i0:i1, j0 + 1:j1 + 1]
+             + src[i0 - 1:i1 - 1, j0:j1] + src[i0 + 1:i1 + 1, j0:j1]) * 0.2
+        out[i0 - r0:i1 - r0, j0 - c0:j1 - c0] = v
+    return out
+
+
+class Problem:
+    def __init__(self, W, H, bs, iters, tol, seed):
+        self.W, self.H, self.bs, self.iters = W, H, bs, iters
+        self.tol2 = tol * tol if tol > 0 else -1.0
+        self.tr, self.tc = -(-H // bs), -(-W // bs)
+        self.n = self.tr * self.tc
+        self.rounds = (iters + 1) // 2
+        g = O.init_grid(W, H, seed)
+        self.A = [g.copy(), g.copy()]
+        gate = 1 if self.tol2 >= 0 else 0
+        self.pend = []
+        for u in range(self.n):
+            deg = len(self.nbrs(u))
+            self.pend.append([0, 1 + deg + gate, 0])
+        self.inflight = self.n
+        self.state = 0
+        self.fix_round = None
+        self.round_tiles = [0, 0]
+        self.res = [[0, 0], [0, 0]]
+        self.s0 = 0
+        self.stop_iters = 0
+        self.s_last = 0
+        self.hash = 0
+        self.done = False
+
+    def nbrs(self, u):
+        ti, tj = divmod(u, self.tc)
+        out = []
+        for di in (-1, 0, 1):
+            for dj in (-1, 0, 1):
+                i, j = ti + di, tj + dj
+                if (di or dj) and 0 <= i < self.tr and 0 <= j < self.tc:
+                    out.append(i * self.tc + j)
+        return out
+
+    def geom(self, u):
+        ti, tj = divmod(u, self.tc)
+        r0, c0 = 1 + ti * self.bs, 1 + tj * self.bs
+        return r0, min(r0 + self.bs, self.H + 1), c0, min(c0 + self.bs, self.W + 1)
+
+
+class Engine:
+    def __init__(self, probs, rng):
+        self.probs = probs
+        self.rng = rng
+        self.queue = []       # ready task words
+        self.running = []     # (task, pending write)
+        for pi, p in enumerate(probs):
+            for u in range(p.n):
+                self.queue.append(("iter", pi, u))
+
+    def push(self, t):
+        self.queue.append(t)
+
+    # ---- task start: the counter re-arm ----
+    def start(self, t):
+        kind, pi, u = t
+        p = self.probs[pi]
+        if kind == "iter":
+            k = p.pend[u][2]
+            gate = 1 if p.tol2 >= 0 else 0
+            p.pend[u][k & 1] = 1 + len(p.nbrs(u)) + gate
+            return k
+        return p.fix_round if kind == "fix" else None
+
+    # ---- the read phase: at the task's start or just before its end ----
+    def read(self, t, k):
+        kind, pi, u = t
+        p = self.probs[pi]
+        r0, r1, c0, c1 = p.geom(u)
+        if kind == "iter":
+            n_sw = min(2, p.iters - 2 * k)
+            src = p.A[k & 1]
+            tol, last = p.tol2 >= 0, k + 1 == p.rounds
+            if n_sw == 1:
+                new = sweep_region(src, r0, r1, c0, c1, p.H, p.W)
+                ra = res_sum(new, src[r0:r1, c0:c1]) if (tol or last) else 0
+                rb = 0
+            else:
+                mid = sweep_region(src, r0 - 1, r1 + 1, c0 - 1, c1 + 1, p.H, p.W)
+                big = src.copy()
+                big[r0 - 1:r1 + 1, c0 - 1:c1 + 1] = mid
+                new = sweep_region(big, r0, r1, c0, c1, p.H, p.W)
+                ra = res_sum(mid[1:-1, 1:-1], src[r0:r1, c0:c1]) if tol else 0
+                rb = res_sum(new, mid[1:-1, 1:-1]) if (tol or last) else 0
+            return (k, new, ra, rb)
+        if kind == "fix":
+            return (k, sweep_region(p.A[k & 1], r0, r1, c0, c1, p.H, p.W), 0, 0)
+        par = ((p.stop_iters + 1) // 2) & 1
+        return (None, O.grid_hash(p.A[par], r0, r1, c0, c1), 0, 0)
+
+    # ---- task end: the write phase and the completion logic ----
+    def finish(self, t, data):
+        kind, pi, u = t
+        p = self.probs[pi]
+        r0, r1, c0, c1 = p.geom(u)
+        k, new, ra, rb = data
+        if kind == "iter":
+            p.A[(k + 1) & 1][r0:r1, c0:c1] = new
+            p.pend[u][2] = k + 1
+            flags = self.complete_round(p, pi, u, k, ra, rb)
+            if flags == "gate" and k + 1 < p.rounds:
+                for v in range(p.n):
+                    p.pend[v][(k + 1) & 1] -= 1
+                    if p.pend[v][(k + 1) & 1] == 0:
+                        p.inflight += 1
+                        self.push(("iter", pi, v))
+        elif kind == "fix":
+            p.A[(k + 1) & 1][r0:r1, c0:c1] = new
+        else:
+            p.hash = (p.hash + new) % 2 ** 64
+        self.finish_task(p, pi)
+
+    def complete_round(self, p, pi, u, k, ra, rb):
+        if k + 1 < p.rounds:
+            for v in [u] + p.nbrs(u):
+                p.pend[v][(k + 1) & 1] -= 1
+                assert p.pend[v][(k + 1) & 1] >= 0
+                if p.pend[v][(k + 1) & 1] == 0:
+                    p.inflight += 1
+                    self.push(("iter", pi, v))
+        tol, last = p.tol2 >= 0, k + 1 == p.rounds
+        if not tol and not last:
+            return None
+        sk = (k & 1) if tol else 0
+        p.res[sk][0] += ra
+        p.res[sk][1] += rb
+        p.round_tiles[sk] += 1
+        if p.round_tiles[sk] != p.n:
+            return None
+        n_sw = min(2, p.iters - 2 * k)
+        a, b = p.res[sk]
+        stop = 0
+        if not tol:
+            stop, s = p.iters, (b if n_sw == 2 else a)
+        else:
+            if k == 0:
+                p.s0 = a
+            thr = p.tol2 * u128d(p.s0)
+            if u128d(a) <= thr or (last and n_sw == 1):
+                stop, s = 2 * k + 1, a
+                if n_sw == 2:
+                    p.fix_round = k
+            elif n_sw == 2 and (u128d(b) <= thr or last):
+                stop, s = 2 * k + 2, b
+        if stop:
+            p.s_last, p.stop_iters, p.state = s, stop, 1
+            return None
+        p.round_tiles[sk] = 0
+        p.res[sk] = [0, 0]
+        return "gate"
+
+    def finish_task(self, p, pi):
+        p.inflight -= 1
+        assert p.inflight >= 0
+        if p.inflight:
+            return
+        if p.state in (1, 4):
+            fix = p.state == 1 and p.fix_round is not None
+            p.state = 4 if fix else 2
+            p.inflight += p.n
+            for v in range(p.n):
+                self.push(("fix" if fix else "hash", pi, v))
+        elif p.state == 2:
+            p.done = True
+        else:
+            raise AssertionError("problem drained while running")
+
+    def run(self, max_overlap):
+        while self.queue or self.running:
+            start = self.queue and (not self.running or
+                                    (len(self.running) < max_overlap and
+                                     self.rng.random() < 0.6))
+            if start:
+                t = self.queue.pop(self.rng.randrange(len(self.queue)))
+                k = self.start(t)
+                early = self.rng.random() < 0.5
+                self.running.append((t, k, self.read(t, k) if early else None))
+            else:
+                t, k, data = self.running.pop(self.rng.randrange(len(self.running)))
+                self.finish(t, data if data is not None else self.read(t, k))
+        assert all(p.done for p in self.probs)
+
+
+CASES = [  # W, H, tile, sweeps, tol
+    (40, 30, 16, 9, 0.0), (33, 17, 16, 6, 0.0), (20, 20, 8, 1, 0.0),
+    (50, 40, 16, 400, 3e-2), (24, 24, 8, 400, 0.5), (30, 18, 8, 7, 1e-9),
+    (17, 33, 16, 400, 1e-2), (12, 12, 4, 2, 0.0)]
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_protocol_matches_oracle_under_random_interleavings(seed):
+    rng = random.Random(seed)
+    probs = [Problem(W, H, bs, it, tol, 100 * seed + i)
+             for i, (W, H, bs, it, tol) in enumerate(CASES)]
+    Engine(probs, rng).run(max_overlap=12)
+    for (W, H, bs, it, tol), p in zip(CASES, probs):
+        s = 100 * seed + CASES.index((W, H, bs, it, tol))
+        if tol > 0:
+            g, n, s_last, s0 = O.jacobi_solve_tol(W, H, it, tol, seed=s)
+        else:
+            (g, s_last), n = O.jacobi_run(W, H, it, seed=s), it
+        assert p.stop_iters == n, (W, H, bs, it, tol)
+        assert p.hash == O.grid_hash(g), (W, H, bs, it, tol)
+        assert p.s_last == s_last, (W, H, bs, it, tol)
+
+
+def test_model_detects_a_missing_corner_dependency(monkeypatch):
+    # the check has teeth: without the diagonal neighbours a fused round can
+    # read a corner cell before (or after) its tile wrote it
+    def sides_only(self, u):
+        ti, tj = divmod(u, self.tc)
+        out = []
+        for di, dj in ((-1, 0), (1, 0), (0, -1), (0, 1)):
+            i, j = ti + di, tj + dj
+            if 0 <= i < self.tr and 0 <= j < self.tc:
+                out.append(i * self.tc + j)
+        return out
+    monkeypatch.setattr(Problem, "nbrs", sides_only)
+    bad = 0
+    for seed in range(6):
+        rng = random.Random(seed)
+        p = Problem(40, 40, 8, 12, 0.0, seed)
+        Engine([p], rng).run(max_overlap=16)
+        g, _ = O.jacobi_run(40, 40, 12, seed=seed)
+        bad += p.hash != O.grid_hash(g)
+    assert bad > 0
