@@ -132,6 +132,7 @@ def jacobi_lib():
         L.chm_jacobi2d_server_poll.argtypes = [vp, C.POINTER(ServerCompletion), C.c_int]
         L.chm_jacobi2d_server_stats.argtypes = [vp, C.POINTER(ServerStats)]
         L.chm_jacobi2d_server_stop.argtypes = [vp, C.c_int, C.c_double]
+        L.chm_device_memory.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
         L.chm_jacobi2d_last_error.restype = C.c_char_p
         L.chm_jacobi2d_abi_version.restype = C.c_int
         _jac = L
@@ -336,3 +337,10 @@ class Server:
 
     def __exit__(self, *a):
         self.stop(abort=a[0] is not None)
+
+
+def device_memory():
+    """(free, total) HBM bytes of the current device."""
+    f, t = C.c_size_t(), C.c_size_t()
+    _check(jacobi_lib().chm_device_memory(C.byref(f), C.byref(t)))
+    return f.value, t.value

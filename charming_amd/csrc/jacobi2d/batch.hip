@@ -59,6 +59,7 @@ __device__ void do_init(const Request& r, const TileGeom& g) {
     const int64_t o = i * r.pitch + kColOff + j;
     r.a[o] = v;
     r.b[o] = v;
+    if (r.c) r.c[o] = v;
   }
 }
 
@@ -84,12 +85,12 @@ __device__ inline void add_residual(uint64_t* slot, jac::u128acc acc) {
   if (__lane_id() == 0 && (acc.lo | acc.hi)) jac::atomic_add_u128(slot, acc);
 }
 
-// One sweep of a tile from array `par` into the other.
+// One sweep of a tile from array `src` into array `dst`.
 template <int NC, int U, bool RES>
-__device__ void do_sweep(const Request& r, const TileGeom& g, int par,
+__device__ void do_sweep(const Request& r, const TileGeom& g, int src, int dst,
                          uint64_t* res_slot) {
-  const double* in = par ? r.b : r.a;
-  double* out = par ? r.a : r.b;
+  const double* in = arr(r, src);
+  double* out = arr(r, dst);
   const TileSplit ts = split_tile<NC>(g);
   const int nW = blockDim.x / 64, wave = threadIdx.x / 64;
   jac::u128acc acc = {0, 0};
@@ -104,13 +105,13 @@ __device__ void do_sweep(const Request& r, const TileGeom& g, int par,
   if (RES) add_residual(res_slot, acc);
 }
 
-// Two fused sweeps of a tile from array `par` into the other; RESA / RESB:
-// residuals of the first / second sweep into res_slot[0..1] / [2..3].
+// Two fused sweeps of a tile from array `src` into array `dst`; RESA /
+// RESB: residuals of the first / second sweep into res_slot[0..1] / [2..3].
 template <int NC, int U, bool RESA, bool RESB>
-__device__ void do_round2(const Request& r, const TileGeom& g, int par,
-                          uint64_t* res_slot) {
-  const double* in = par ? r.b : r.a;
-  double* out = par ? r.a : r.b;
+__device__ void do_round2(const Request& r, const TileGeom& g, int src,
+                          int dst, uint64_t* res_slot) {
+  const double* in = arr(r, src);
+  double* out = arr(r, dst);
   const TileSplit ts = split_tile<NC>(g);
   const int nW = blockDim.x / 64, wave = threadIdx.x / 64;
   jac::u128acc acc_a = {0, 0}, acc_b = {0, 0};
@@ -127,9 +128,9 @@ __device__ void do_round2(const Request& r, const TileGeom& g, int par,
   if (RESB) add_residual(res_slot + 2, acc_b);
 }
 
-// Hash of the tile's interior cells of array `par`.
-__device__ uint64_t do_hash(const Request& r, const TileGeom& g, int par) {
-  const double* u = par ? r.b : r.a;
+// Hash of the tile's interior cells of array `src`.
+__device__ uint64_t do_hash(const Request& r, const TileGeom& g, int src) {
+  const double* u = arr(r, src);
   const int64_t cols = g.c1 - g.c0, n = (int64_t)(g.r1 - g.r0) * cols;
   uint64_t h = 0;
   for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
@@ -164,20 +165,21 @@ __device__ void release_deps(const EngineDesc& E, Prob& p, const Request& r,
   for (int i = 0; i < n; i++) push_task(E, task_word(K_ITER, slot, ready[i]));
 }
 
-// Thread 0, after round rr of `tile`: release round rr+1 of the tile and its
-// neighbours; the last tile of a round whose residual matters decides
-// whether the problem stops.  Returns F_GATE when the decision lets round
-// rr+1 of every tile go (its gate is then dropped by the whole workgroup).
+// Thread 0: count the tile in round rr; the last tile of a round whose
+// residual matters decides whether the problem stops.  Returns F_GATE when
+// the decision lets round rr+2 of every tile go (its gate is then dropped
+// by the whole workgroup).
 //
-// With a tolerance every round waits for the decision on the previous one,
-// so decisions are made in order and the arrays of round rr are intact when
-// it is decided: a stop after the first sweep of a fused round (whose result
-// was not stored) is repaired by redoing that sweep alone (K_FIX tasks).
-// The rule is the monolithic tolerance solve's: stop after the first sweep
-// k with S_k <= tol^2 * S_0, in the same doubles.
-__device__ int complete_round(const EngineDesc& E, Prob& p, const Request& r,
-                              uint32_t slot, uint32_t tile, int rr) {
-  if (rr + 1 < r.rounds) release_deps(E, p, r, slot, tile, (rr + 1) & 1);
+// With a tolerance, round rr+2 waits for the decision on round rr, and the
+// decision comes before the release of round rr+1 (complete_round), so
+// decisions are made in order and round rr's input (array rr % 3) is intact
+// when it is decided even if round rr+1 is running.  Stop after the first
+// sweep k with S_k <= tol^2 * S_0 (the rule and the doubles of the
+// monolithic tolerance solve): if k is the second sweep of the round, its
+// output is the result; if it is the first, whose output the fused round
+// never stored, every tile redoes that sweep alone from the intact input
+// (K_FIX tasks).
+__device__ int decide_round(Prob& p, const Request& r, int rr) {
   const bool tol = r.tol2 >= 0;
   const bool last = rr + 1 == r.rounds;
   if (!tol && !last) return 0;
@@ -185,6 +187,7 @@ __device__ int complete_round(const EngineDesc& E, Prob& p, const Request& r,
   const int sk = tol ? (rr & 1) : 0;
   if (aadd(&p.round_tiles[sk], 1u) != n_tiles - 1) return 0;
   // every tile has finished round rr: its residuals are complete
+  if (ald(&p.state) != 0) return 0;  // stopped on the previous round
   const int n_sw = min(2, r.max_iters - 2 * rr);
   const uint64_t alo = ald(&p.res[sk][0][0]), ahi = ald(&p.res[sk][0][1]);
   const uint64_t blo = ald(&p.res[sk][1][0]), bhi = ald(&p.res[sk][1][1]);
@@ -230,7 +233,16 @@ __device__ int complete_round(const EngineDesc& E, Prob& p, const Request& r,
     }
   }
   release_agent();
-  return stop ? 0 : F_GATE;
+  return (!stop && rr + 2 < r.rounds) ? F_GATE : 0;
+}
+
+// Thread 0, after round rr of `tile`: decide, then release round rr+1 of
+// the tile and its neighbours.
+__device__ int complete_round(const EngineDesc& E, Prob& p, const Request& r,
+                              uint32_t slot, uint32_t tile, int rr) {
+  const int f = decide_round(p, r, rr);
+  if (rr + 1 < r.rounds) release_deps(E, p, r, slot, tile, (rr + 1) & 1);
+  return f;
 }
 
 // Thread 0: a task of the problem is finished.  The last task of a stopped
@@ -288,6 +300,7 @@ __device__ void admit(const EngineDesc& E, uint64_t idx, Request* s_req) {
     Request q;
     q.a = vr->a;
     q.b = vr->b;
+    q.c = vr->c;
     q.pend = vr->pend;
     q.pitch = vr->pitch;
     q.W = vr->W;
@@ -329,12 +342,11 @@ __device__ void admit(const EngineDesc& E, uint64_t idx, Request* s_req) {
   __syncthreads();
   const Request q = uni(*s_req);
   const uint32_t n_tiles = (uint32_t)(q.tr * q.tc);
-  const uint32_t gate = q.tol2 >= 0 ? 1u : 0u;  // round 1 waits for decision 0
   for (uint32_t u = threadIdx.x; u < n_tiles; u += blockDim.x) {
     uint32_t nb[8];
     const uint32_t deg = (uint32_t)tile_neighbors(q, u, nb);
     q.pend[3 * u + 0] = q.init_on_device ? 1 + deg : 0;  // round 0
-    q.pend[3 * u + 1] = 1 + deg + gate;                  // round 1
+    q.pend[3 * u + 1] = 1 + deg;                         // round 1
     q.pend[3 * u + 2] = 0;                               // rounds done
   }
   drain();
@@ -408,18 +420,18 @@ __device__ void run_round(const Request& r, const TileGeom& g, int rr,
   const bool last = rr + 1 == r.rounds;
   const int n_sw = min(2, r.max_iters - 2 * rr);
   uint64_t* slot = &p.res[tol ? (rr & 1) : 0][0][0];
-  const int par = rr & 1;
+  const int src = rr % 3, dst = (rr + 1) % 3;
   if (n_sw == 1) {
     if (tol || last)
-      do_sweep<NC, U, true>(r, g, par, slot);
+      do_sweep<NC, U, true>(r, g, src, dst, slot);
     else
-      do_sweep<NC, U, false>(r, g, par, slot);
+      do_sweep<NC, U, false>(r, g, src, dst, slot);
   } else if (tol) {
-    do_round2<NC, U, true, true>(r, g, par, slot);
+    do_round2<NC, U, true, true>(r, g, src, dst, slot);
   } else if (last) {
-    do_round2<NC, U, false, true>(r, g, par, slot);
+    do_round2<NC, U, false, true>(r, g, src, dst, slot);
   } else {
-    do_round2<NC, U, false, false>(r, g, par, slot);
+    do_round2<NC, U, false, false>(r, g, src, dst, slot);
   }
 }
 
@@ -455,7 +467,7 @@ __global__ void __launch_bounds__(T) engine_kernel(EngineDesc E) {
       if (kind == K_ITER) {
         k = (int)ald(&s_r.pend[3 * tile + 2]);
         // re-arm this parity's counter for round k+2: the tile's and its
-        // neighbours' round k+1, and the stop decision on round k+1
+        // neighbours' round k+1, and the stop decision on round k
         uint32_t nb[8];
         const uint32_t deg = (uint32_t)tile_neighbors(s_r, tile, nb);
         const uint32_t gate = s_r.tol2 >= 0 ? 1u : 0u;
@@ -476,10 +488,10 @@ __global__ void __launch_bounds__(T) engine_kernel(EngineDesc E) {
     } else if (kind == K_ITER) {
       run_round<NC, U>(r, g, k, p);
     } else if (kind == K_FIX) {
-      do_sweep<NC, U, false>(r, g, k & 1, nullptr);
+      do_sweep<NC, U, false>(r, g, k % 3, (k + 1) % 3, nullptr);
     } else {
-      // the result of `k` sweeps is in array ceil(k / 2) & 1
-      const uint64_t h = do_hash(r, g, ((k + 1) / 2) & 1);
+      // the result of `k` sweeps is in array ceil(k / 2) % 3
+      const uint64_t h = do_hash(r, g, ((k + 1) / 2) % 3);
       if (__lane_id() == 0 && h) aadd(&p.hash, h);
     }
     drain();
@@ -502,10 +514,10 @@ __global__ void __launch_bounds__(T) engine_kernel(EngineDesc E) {
     __syncthreads();
     const uint32_t n_tiles = (uint32_t)(r.tr * r.tc);
     if (uni(s_flags) & F_GATE) {
-      // the decision on round k lets round k+1 (parity (k+1) & 1) go
-      if (k + 1 < r.rounds) {
+      // the decision on round k lets round k+2 (parity k & 1) go
+      if (k + 2 < r.rounds) {
         for (uint32_t u = threadIdx.x; u < n_tiles; u += blockDim.x)
-          if (asub(&r.pend[3 * u + ((k + 1) & 1)], 1u) == 1) {
+          if (asub(&r.pend[3 * u + (k & 1)], 1u) == 1) {
             aadd(&p.inflight, 1u);
             push_task(E, task_word(K_ITER, slot, u),
                       (int)((blockIdx.x + u) % (uint32_t)E.n_wg));
@@ -546,6 +558,7 @@ __global__ void init_problem(Request r) {
     const double v = (j >= -1 && j <= r.W + 2) ? u0_cell(r, i, j) : 0.0;
     r.a[e - r.pitch] = v;
     r.b[e - r.pitch] = v;
+    if (r.c) r.c[e - r.pitch] = v;
   }
 }
 
@@ -857,8 +870,10 @@ static void store_out(const BatchOut& out, int i, const Completion& c) {
 }
 
 // Lays out the grids of n problems in one arena.
+// n_arr: 3 for the engine (rounds rotate over three arrays), 2 for the
+// plain form.
 static int layout_batch(const chm_batch_problem* probs, int n, int block,
-                        int iters_cap, const int* blocks,
+                        int iters_cap, const int* blocks, int n_arr,
                         std::vector<Built>& bl, size_t* total, uint64_t* tiles) {
   bl.resize(n);
   *total = 0;
@@ -869,13 +884,13 @@ static int layout_batch(const chm_batch_problem* probs, int n, int block,
     if (blocks && blocks[i] > 0) pb.block = blocks[i];
     CHM_TRY(check_problem(pb, block));
     bl[i] = build_request(pb, block);
-    *total += 2 * bl[i].grid_bytes + bl[i].pend_bytes;
+    *total += n_arr * bl[i].grid_bytes + bl[i].pend_bytes;
     *tiles += (uint64_t)bl[i].r.tr * bl[i].r.tc;
   }
   return CHM_OK;
 }
 
-static void place_batch(std::vector<Built>& bl, char* arena) {
+static void place_batch(std::vector<Built>& bl, char* arena, int n_arr) {
   size_t off = 0;
   for (size_t i = 0; i < bl.size(); i++) {
     Request& r = bl[i].r;
@@ -883,6 +898,11 @@ static void place_batch(std::vector<Built>& bl, char* arena) {
     off += bl[i].grid_bytes;
     r.b = (double*)(arena + off) + r.pitch;
     off += bl[i].grid_bytes;
+    r.c = nullptr;
+    if (n_arr == 3) {
+      r.c = (double*)(arena + off) + r.pitch;
+      off += bl[i].grid_bytes;
+    }
     r.pend = (uint32_t*)(arena + off);
     off += bl[i].pend_bytes;
     r.init_on_device = 0;
@@ -906,14 +926,15 @@ static int batch_once(const chm_batch_problem* probs, int n,
   std::vector<Built> bl;
   size_t total = 0;
   uint64_t tiles = 0;
-  CHM_TRY(layout_batch(probs, n, block, iters_cap, blocks, bl, &total, &tiles));
+  CHM_TRY(layout_batch(probs, n, block, iters_cap, blocks, 3, bl, &total,
+                       &tiles));
   Engine e;
   int rc = engine_setup(e, threads, nc, u, (uint32_t)n, tiles, false);
   char* arena = nullptr;
   if (rc == CHM_OK && hipMalloc(&arena, total) != hipSuccess)
     rc = fail(CHM_E_NOMEM, "batch grids: %zu bytes", total);
   if (rc == CHM_OK) {
-    place_batch(bl, arena);
+    place_batch(bl, arena, 3);
     std::vector<Request> reqs(n);
     for (int i = 0; i < n; i++) {
       init_problem<<<512, 256, 0, e.stream>>>(bl[i].r);
@@ -1146,6 +1167,20 @@ struct Server {
   PlainCtx pc;
 };
 
+// Tile side of a served request when none is given: a lone request is cut
+// into about as many tiles as there are workgroups, so one round of it
+// occupies the whole GPU; with k requests resident each gets about 1/(k+1)
+// of them, and tiles grow (fewer fences per cell) as the load rises.
+static int serve_block(int W, int H, int n_wg, int resident) {
+  const int64_t target = std::max<int64_t>(1, n_wg / (1 + resident));
+  static const int kSides[] = {512, 256, 128};
+  for (int bs : kSides) {
+    const int64_t tiles = (int64_t)((W + bs - 1) / bs) * ((H + bs - 1) / bs);
+    if (tiles >= target) return bs;
+  }
+  return 64;
+}
+
 static bool pool_alloc(Server& s, size_t bytes, size_t* off) {
   for (auto it = s.free_blocks.begin(); it != s.free_blocks.end(); ++it) {
     if (it->second >= bytes) {
@@ -1217,6 +1252,7 @@ static void server_free(Server* s) {
   if (s->pool) (void)hipFree(s->pool);
   s->pool = nullptr;
   engine_free(s->e);
+  (void)hipDeviceSynchronize();
   delete s;
 }
 
@@ -1248,7 +1284,7 @@ int chm_jacobi2d_batch_plain(const chm_batch_problem* probs, int n,
   std::vector<Built> bl;
   size_t total = 0;
   uint64_t tiles = 0;
-  CHM_TRY(layout_batch(probs, n, 0, 0, nullptr, bl, &total, &tiles));
+  CHM_TRY(layout_batch(probs, n, 0, 0, nullptr, 2, bl, &total, &tiles));
   PlainCtx c;
   int rc = plain_ctx_init(c);
   char* arena = nullptr;
@@ -1257,7 +1293,7 @@ int chm_jacobi2d_batch_plain(const chm_batch_problem* probs, int n,
   if (rc == CHM_OK && hipMalloc(&arena, total) != hipSuccess)
     rc = fail(CHM_E_NOMEM, "batch grids: %zu bytes", total);
   if (rc == CHM_OK) {
-    place_batch(bl, arena);
+    place_batch(bl, arena, 2);
     for (int i = 0; i < n; i++) init_problem<<<512, 256, 0, c.s>>>(bl[i].r);
     if (hipStreamSynchronize(c.s) != hipSuccess)
       rc = fail(CHM_E_HIP, "seeding the batch failed");
@@ -1406,12 +1442,7 @@ int chm_jacobi2d_server_start(const chm_server_config* cfg, void** handle) {
   if (!cfg || !handle) return fail(CHM_E_INVALID, "null argument");
   if (cfg->mode != CHM_SERVE_ENGINE && cfg->mode != CHM_SERVE_PLAIN)
     return fail(CHM_E_INVALID, "server mode %d", cfg->mode);
-  size_t fr = 0, tot = 0;
-  CHM_TRY_HIP(hipMemGetInfo(&fr, &tot));
   const size_t budget = cfg->budget_bytes ? cfg->budget_bytes : (size_t)16 << 30;
-  if (budget + ((size_t)2 << 30) > fr)
-    return fail(CHM_E_NOMEM, "budget %zu bytes exceeds free HBM %zu - 2 GiB",
-                budget, fr);
   Server* s = new Server();
   s->mode = cfg->mode;
   s->block = cfg->block;
@@ -1429,8 +1460,14 @@ int chm_jacobi2d_server_start(const chm_server_config* cfg, void** handle) {
   } else {
     rc = plain_ctx_init(s->pc);
   }
-  if (rc == CHM_OK && hipMalloc(&s->pool, budget) != hipSuccess)
-    rc = fail(CHM_E_NOMEM, "serving pool of %zu bytes", budget);
+  // the budget is reserved up front: admission never allocates
+  if (rc == CHM_OK && hipMalloc(&s->pool, budget) != hipSuccess) {
+    (void)hipGetLastError();
+    size_t fr = 0, tot = 0;
+    (void)hipMemGetInfo(&fr, &tot);
+    rc = fail(CHM_E_NOMEM, "serving pool of %zu bytes (free HBM %zu of %zu)",
+              budget, fr, tot);
+  }
   if (rc == CHM_OK) {
     s->pool_size = budget;
     s->free_blocks[0] = budget;
@@ -1460,10 +1497,13 @@ int chm_jacobi2d_server_submit(void* handle, const chm_batch_problem* pb,
                                uint64_t* ticket) {
   Server* s = (Server*)handle;
   if (!s || !pb || !ticket) return fail(CHM_E_INVALID, "null argument");
-  CHM_TRY(check_problem(*pb, s->block));
-  Built b = build_request(*pb, s->block);
   const bool engine = s->mode == CHM_SERVE_ENGINE;
-  const size_t bytes = 2 * b.grid_bytes + (engine ? b.pend_bytes : 0);
+  int block = s->block;
+  if (engine && pb->block <= 0 && block <= 0)
+    block = serve_block(pb->width, pb->height, s->e.d.n_wg, (int)s->live.size());
+  CHM_TRY(check_problem(*pb, block));
+  Built b = build_request(*pb, block);
+  const size_t bytes = (engine ? 3 : 2) * b.grid_bytes + (engine ? b.pend_bytes : 0);
   if (bytes > s->pool_size)
     return fail(CHM_E_INVALID, "request needs %zu bytes, budget %zu", bytes,
                 s->pool_size);
@@ -1476,7 +1516,8 @@ int chm_jacobi2d_server_submit(void* handle, const chm_batch_problem* pb,
   Request& r = b.r;
   r.a = (double*)(s->pool + off) + r.pitch;  // row 0 (row -1 comes first)
   r.b = (double*)(s->pool + off + b.grid_bytes) + r.pitch;
-  r.pend = engine ? (uint32_t*)(s->pool + off + 2 * b.grid_bytes) : nullptr;
+  r.c = engine ? (double*)(s->pool + off + 2 * b.grid_bytes) + r.pitch : nullptr;
+  r.pend = engine ? (uint32_t*)(s->pool + off + 3 * b.grid_bytes) : nullptr;
   r.init_on_device = 1;
   r.ticket = s->next_ticket++;
   r.slot = s->free_slots.back();

@@ -9,7 +9,7 @@
 // counter of unmet dependencies (the tile's own previous round, each of its
 // up to eight neighbours' previous round - a fused round reads two cells
 // beyond the tile, corners included - and, when the problem stops on a
-// residual, the stop decision on the previous round) is decremented by every
+// residual, the stop decision two rounds back) is decremented by every
 // producer; whoever takes it to zero enqueues the task.  Halos are read in
 // place from the neighbours' cells (the zero-copy halo mode: the message
 // carries no payload).  Ready tasks go to the completing workgroup's own
@@ -39,12 +39,15 @@ __host__ __device__ inline uint32_t task_tile(uint32_t w) {
   return w & (kMaxTiles - 1);
 }
 
-// A request as the host publishes it.  Arrays have halos two deep (rows
-// -1 .. H+2); a and b point at row 0.  Round r reads array r & 1 (a = 0,
-// b = 1) and writes the other.
+// A request as the host publishes it.  Three arrays with halos two deep
+// (rows -1 .. H+2), pointers at row 0.  Round r reads array r % 3 and
+// writes array (r + 1) % 3, so the input of round r survives round r+1:
+// a stop decided on round r while round r+1 runs can still redo a sweep
+// from it.
 struct Request {
   double* a;
   double* b;
+  double* c;
   uint32_t* pend;     // 3 words per tile: dependency counters of the next
                       // even and odd round, rounds done
   int64_t pitch;

@@ -77,6 +77,7 @@ __device__ inline Request uni(const Request& s) {
   Request r;
   r.a = uni(s.a);
   r.b = uni(s.b);
+  r.c = uni(s.c);
   r.pend = uni(s.pend);
   r.pitch = uni(s.pitch);
   r.W = uni(s.W);
@@ -94,6 +95,11 @@ __device__ inline Request uni(const Request& s) {
   r.slot = uni(s.slot);
   r.rounds = uni(s.rounds);
   return r;
+}
+
+// Array i (0, 1, 2) of a request.
+__device__ inline double* arr(const Request& r, int i) {
+  return i == 0 ? r.a : (i == 1 ? r.b : r.c);
 }
 
 // ---------------------------------------------------------------------------

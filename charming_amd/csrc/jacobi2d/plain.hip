@@ -418,6 +418,12 @@ int chm_jacobi2d_phase_stats(void* handle, double* out8) {
   return persistent_phase_stats(h, out8);
 }
 
+int chm_device_memory(size_t* free_bytes, size_t* total_bytes) {
+  if (!free_bytes || !total_bytes) return fail(CHM_E_INVALID, "null argument");
+  CHM_TRY_HIP(hipMemGetInfo(free_bytes, total_bytes));
+  return CHM_OK;
+}
+
 int chm_hbm_copy_probe(size_t bytes, int reps, double* ms_per_copy) {
   if (bytes < 16 || reps < 1 || !ms_per_copy)
     return fail(CHM_E_INVALID, "bad copy probe arguments");

@@ -113,6 +113,10 @@ const char* chm_jacobi2d_last_error(void);
  * it calibrates the roofline and traffic figures of the sweep. */
 int chm_hbm_copy_probe(size_t bytes, int reps, double* ms_per_copy);
 
+/* Free and total HBM of the current device, in bytes (what an admission
+ * budget can be sized from).  No reference counterpart. */
+int chm_device_memory(size_t* free_bytes, size_t* total_bytes);
+
 /* Sanity hook with no GPU work: returns the ABI version. */
 int chm_jacobi2d_abi_version(void);
 
@@ -192,7 +196,9 @@ typedef struct chm_server_config {
   int mode;            /* enum chm_server_mode */
   size_t budget_bytes; /* HBM for resident grids (0 = 16 GiB) */
   int max_active;      /* requests resident at once (0 = 4096) */
-  int wg_threads, block, panel_chunks, rows_in_flight; /* as chm_batch_config */
+  int wg_threads, block, panel_chunks, rows_in_flight; /* as chm_batch_config;
+                          block 0 = by load: a request gets about
+                          workgroups / (1 + resident requests) tiles */
 } chm_server_config;
 
 typedef struct chm_server_completion {

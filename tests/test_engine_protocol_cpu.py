@@ -58,12 +58,11 @@ class Problem:
         self.n = self.tr * self.tc
         self.rounds = (iters + 1) // 2
         g = O.init_grid(W, H, seed)
-        self.A = [g.copy(), g.copy()]
-        gate = 1 if self.tol2 >= 0 else 0
+        self.A = [g.copy(), g.copy(), g.copy()]
         self.pend = []
         for u in range(self.n):
             deg = len(self.nbrs(u))
-            self.pend.append([0, 1 + deg + gate, 0])
+            self.pend.append([0, 1 + deg, 0])
         self.inflight = self.n
         self.state = 0
         self.fix_round = None
@@ -110,7 +109,7 @@ class Engine:
         p = self.probs[pi]
         if kind == "iter":
             k = p.pend[u][2]
-            gate = 1 if p.tol2 >= 0 else 0
+            gate = 1 if p.tol2 >= 0 else 0  # round k+2 waits for decision k
             p.pend[u][k & 1] = 1 + len(p.nbrs(u)) + gate
             return k
         return p.fix_round if kind == "fix" else None
@@ -122,7 +121,7 @@ class Engine:
         r0, r1, c0, c1 = p.geom(u)
         if kind == "iter":
             n_sw = min(2, p.iters - 2 * k)
-            src = p.A[k & 1]
+            src = p.A[k % 3]
             tol, last = p.tol2 >= 0, k + 1 == p.rounds
             if n_sw == 1:
                 new = sweep_region(src, r0, r1, c0, c1, p.H, p.W)
@@ -137,9 +136,9 @@ class Engine:
                 rb = res_sum(new, mid[1:-1, 1:-1]) if (tol or last) else 0
             return (k, new, ra, rb)
         if kind == "fix":
-            return (k, sweep_region(p.A[k & 1], r0, r1, c0, c1, p.H, p.W), 0, 0)
-        par = ((p.stop_iters + 1) // 2) & 1
-        return (None, O.grid_hash(p.A[par], r0, r1, c0, c1), 0, 0)
+            return (k, sweep_region(p.A[k % 3], r0, r1, c0, c1, p.H, p.W), 0, 0)
+        res_arr = ((p.stop_iters + 1) // 2) % 3
+        return (None, O.grid_hash(p.A[res_arr], r0, r1, c0, c1), 0, 0)
 
     # ---- task end: the write phase and the completion logic ----
     def finish(self, t, data):
@@ -148,22 +147,23 @@ class Engine:
         r0, r1, c0, c1 = p.geom(u)
         k, new, ra, rb = data
         if kind == "iter":
-            p.A[(k + 1) & 1][r0:r1, c0:c1] = new
+            p.A[(k + 1) % 3][r0:r1, c0:c1] = new
             p.pend[u][2] = k + 1
             flags = self.complete_round(p, pi, u, k, ra, rb)
-            if flags == "gate" and k + 1 < p.rounds:
+            if flags == "gate" and k + 2 < p.rounds:
                 for v in range(p.n):
-                    p.pend[v][(k + 1) & 1] -= 1
-                    if p.pend[v][(k + 1) & 1] == 0:
+                    p.pend[v][k & 1] -= 1
+                    if p.pend[v][k & 1] == 0:
                         p.inflight += 1
                         self.push(("iter", pi, v))
         elif kind == "fix":
-            p.A[(k + 1) & 1][r0:r1, c0:c1] = new
+            p.A[(k + 1) % 3][r0:r1, c0:c1] = new
         else:
             p.hash = (p.hash + new) % 2 ** 64
         self.finish_task(p, pi)
 
     def complete_round(self, p, pi, u, k, ra, rb):
+        flags = self.decide(p, k, ra, rb)
         if k + 1 < p.rounds:
             for v in [u] + p.nbrs(u):
                 p.pend[v][(k + 1) & 1] -= 1
@@ -171,6 +171,9 @@ class Engine:
                 if p.pend[v][(k + 1) & 1] == 0:
                     p.inflight += 1
                     self.push(("iter", pi, v))
+        return flags
+
+    def decide(self, p, k, ra, rb):
         tol, last = p.tol2 >= 0, k + 1 == p.rounds
         if not tol and not last:
             return None
@@ -180,6 +183,8 @@ class Engine:
         p.round_tiles[sk] += 1
         if p.round_tiles[sk] != p.n:
             return None
+        if p.state != 0:
+            return None  # a stop was decided on the previous round
         n_sw = min(2, p.iters - 2 * k)
         a, b = p.res[sk]
         stop = 0
@@ -240,7 +245,7 @@ CASES = [  # W, H, tile, sweeps, tol
     (17, 33, 16, 400, 1e-2), (12, 12, 4, 2, 0.0)]
 
 
-@pytest.mark.parametrize("seed", [1, 2, 3])
+@pytest.mark.parametrize("seed", [1, 2, 3, 4, 5, 6])
 def test_protocol_matches_oracle_under_random_interleavings(seed):
     rng = random.Random(seed)
     probs = [Problem(W, H, bs, it, tol, 100 * seed + i)

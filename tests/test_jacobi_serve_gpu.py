@@ -81,10 +81,10 @@ def test_result_alone_equals_in_mix(cam):
 
 
 def test_admission_against_budget(cam):
-    # a 1024^2 request needs two arrays of 1026 x 1056 doubles (17.3 MB);
-    # a 24 MiB budget holds one
+    # a 1024^2 request needs three arrays of 1028 x 1056 doubles (26 MB);
+    # a 40 MiB budget holds one
     p = {"width": 1024, "height": 1024, "iters": 200, "tol": 1e-9, "seed": 3}
-    with cam.Server("engine", budget_bytes=24 << 20) as srv:
+    with cam.Server("engine", budget_bytes=40 << 20) as srv:
         t1 = srv.submit(p)
         assert t1 is not None
         assert srv.submit(dict(p, seed=4)) is None  # BUSY: budget in use
