@@ -76,7 +76,7 @@ def cpu_baseline(budget_s):
                         want_grid=False)
     t = O.last_run_seconds()
     # scale the iteration count so the timed sample is about budget_s
-    iters = max(2, min(200, int(budget_s / max(t / 2, 1e-6))))
+    iters = max(2, min(ITERS, int(budget_s / max(t / 2, 1e-6))))
     O.jacobi_chares_run(W, H, CHARES[0], CHARES[1], iters, cores, seed=SEED,
                         want_grid=False)
     t = O.last_run_seconds()

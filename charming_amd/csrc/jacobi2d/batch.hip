@@ -80,9 +80,25 @@ __device__ inline TileSplit split_tile(const TileGeom& g) {
   return t;
 }
 
-__device__ inline void add_residual(uint64_t* slot, jac::u128acc acc) {
+// Workgroup-collective: the exact sum of every lane's u128 partials (waves
+// reduce across lanes, then thread 0 across waves in LDS) goes to the
+// problem's accumulator with one 128-bit atomic add per task, not one per
+// wave: thousands of tiles add into the same two words every round.
+__device__ void add_residual(uint64_t* slot, jac::u128acc acc) {
+  __shared__ uint64_t part[16][2];
   acc = jac::wave_reduce_u128(acc);
-  if (__lane_id() == 0 && (acc.lo | acc.hi)) jac::atomic_add_u128(slot, acc);
+  const int wave = threadIdx.x / 64, n_waves = blockDim.x / 64;
+  if (__lane_id() == 0) {
+    part[wave][0] = acc.lo;
+    part[wave][1] = acc.hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    jac::u128acc t = {0, 0};
+    for (int w = 0; w < n_waves; w++) jac::acc_add(t, jac::u128acc{part[w][0], part[w][1]});
+    if (t.lo | t.hi) jac::atomic_add_u128(slot, t);
+  }
+  __syncthreads();
 }
 
 // One sweep of a tile from array `src` into array `dst`.
@@ -150,19 +166,23 @@ __host__ __device__ inline double u128d(uint64_t lo, uint64_t hi) {
   return ldexp((double)hi, 64) + (double)lo;
 }
 
-// Thread 0: decrement the counters (word `word`) of `tile` and its
-// neighbours; queue the round tasks this makes ready.
+// Wave 0: decrement the counters (word `word`) of `tile` and its up to
+// eight neighbours, one lane each so the atomics overlap; a lane that takes
+// a counter to zero queues that round task (counted in flight first).
 __device__ void release_deps(const EngineDesc& E, Prob& p, const Request& r,
                              uint32_t slot, uint32_t tile, int word) {
-  uint32_t nb[8], ready[9];
-  const int deg = tile_neighbors(r, tile, nb);
-  int n = 0;
-  if (asub(&r.pend[3 * tile + word], 1u) == 1) ready[n++] = tile;
-  for (int i = 0; i < deg; i++)
-    if (asub(&r.pend[3 * nb[i] + word], 1u) == 1) ready[n++] = nb[i];
-  if (n == 0) return;
-  aadd(&p.inflight, (uint32_t)n);
-  for (int i = 0; i < n; i++) push_task(E, task_word(K_ITER, slot, ready[i]));
+  const int lane = __lane_id();
+  const int ti = (int)tile / r.tc, tj = (int)tile % r.tc;
+  int target = -1;
+  if (lane < 9) {
+    const int i = ti + lane / 3 - 1, j = tj + lane % 3 - 1;  // lane 4: the tile
+    if (i >= 0 && i < r.tr && j >= 0 && j < r.tc) target = i * r.tc + j;
+  }
+  if (target >= 0 && asub(&r.pend[3 * target + word], 1u) == 1) {
+    aadd(&p.inflight, 1u);
+    push_task(E, task_word(K_ITER, slot, (uint32_t)target),
+              (int)((blockIdx.x + (uint32_t)lane) % (uint32_t)E.n_wg));
+  }
 }
 
 // Thread 0: count the tile in round rr; the last tile of a round whose
@@ -171,7 +191,7 @@ __device__ void release_deps(const EngineDesc& E, Prob& p, const Request& r,
 // by the whole workgroup).
 //
 // With a tolerance, round rr+2 waits for the decision on round rr, and the
-// decision comes before the release of round rr+1 (complete_round), so
+// decision comes before the release of round rr+1 (engine_kernel), so
 // decisions are made in order and round rr's input (array rr % 3) is intact
 // when it is decided even if round rr+1 is running.  Stop after the first
 // sweep k with S_k <= tol^2 * S_0 (the rule and the doubles of the
@@ -234,15 +254,6 @@ __device__ int decide_round(Prob& p, const Request& r, int rr) {
   }
   release_agent();
   return (!stop && rr + 2 < r.rounds) ? F_GATE : 0;
-}
-
-// Thread 0, after round rr of `tile`: decide, then release round rr+1 of
-// the tile and its neighbours.
-__device__ int complete_round(const EngineDesc& E, Prob& p, const Request& r,
-                              uint32_t slot, uint32_t tile, int rr) {
-  const int f = decide_round(p, r, rr);
-  if (rr + 1 < r.rounds) release_deps(E, p, r, slot, tile, (rr + 1) & 1);
-  return f;
 }
 
 // Thread 0: a task of the problem is finished.  The last task of a stopped
@@ -500,10 +511,8 @@ __global__ void __launch_bounds__(T) engine_kernel(EngineDesc E) {
       int f = 0;
       if (kind == K_ITER) ast(&r.pend[3 * tile + 2], (uint32_t)(k + 1));
       release_agent();
-      if (kind == K_INIT) {
-        release_deps(E, p, r, slot, tile, 0);
-      } else if (kind == K_ITER) {
-        f = complete_round(E, p, r, slot, tile, k);
+      if (kind == K_ITER) {
+        f = decide_round(p, r, k);
         const uint64_t sweeps = (uint64_t)min(2, r.max_iters - 2 * k);
         aadd(&E.ctl[CW_STATS + 3],
              sweeps * (uint64_t)(g.r1 - g.r0) * (uint64_t)(g.c1 - g.c0));
@@ -512,6 +521,13 @@ __global__ void __launch_bounds__(T) engine_kernel(EngineDesc E) {
       s_flags = f;
     }
     __syncthreads();
+    // after the decision: release the tile's and its neighbours' next round
+    if (threadIdx.x < 64) {
+      const int word = kind == K_INIT ? 0
+                       : (kind == K_ITER && k + 1 < r.rounds) ? ((k + 1) & 1)
+                                                              : -1;
+      if (word >= 0) release_deps(E, p, r, slot, tile, word);
+    }
     const uint32_t n_tiles = (uint32_t)(r.tr * r.tc);
     if (uni(s_flags) & F_GATE) {
       // the decision on round k lets round k+2 (parity k & 1) go
@@ -523,8 +539,8 @@ __global__ void __launch_bounds__(T) engine_kernel(EngineDesc E) {
                       (int)((blockIdx.x + u) % (uint32_t)E.n_wg));
           }
       }
-      drain();
     }
+    drain();  // the counts of queued tasks land before this task's own
     __syncthreads();
     if (threadIdx.x == 0) s_flags = finish_task(p, n_tiles);
     __syncthreads();
