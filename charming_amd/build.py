@@ -39,6 +39,7 @@ LIBS = [
     ("libchm_hello.so", ["examples/hello.hip"], ["charming_amd"]),
     ("libchm_refnum.so", ["examples/refnum.hip"], ["charming_amd"]),
     ("libchm_pingpong.so", ["examples/pingpong.hip"], ["charming_amd"]),
+    ("libchm_async_wait.so", ["examples/async_wait.hip"], ["charming_amd"]),
 ]
 
 # chare-program executables: the program's TU + the library-provided main
@@ -46,6 +47,7 @@ EXES = [
     ("hello", ["examples/hello.hip"]),
     ("refnum", ["examples/refnum.hip"]),
     ("pingpong", ["examples/pingpong.hip"]),
+    ("async_wait", ["examples/async_wait.hip"]),
 ]
 MAIN_SRC = "runtime/main.cpp"
 BINDIR = os.path.join(PKG, "bin")

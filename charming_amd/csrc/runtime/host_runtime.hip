@@ -57,6 +57,7 @@ const char* err_name(uint32_t c) {
     case charm::rt::ERR_SPIN_LIMIT: return "send spin limit reached (inbox full)";
     case charm::rt::ERR_BAD_CHARE: return "bad chare index, entry method or table";
     case charm::rt::ERR_UNSUPPORTED: return "unsupported feature";
+    case charm::rt::ERR_WAITS_FULL: return "too many pending async_waits on a PE";
   }
   return "unknown";
 }

@@ -42,8 +42,13 @@ struct entry_method {
   static __device__ void call(void* obj, void* arg) { Func(*(C*)obj, arg); }
 };
 
-// Kept for source compatibility (chare.h:41-58); async_wait is a multi-GPU
-// feature of the reference and is not part of this one-GPU runtime.
+// Comparison operators of async_wait, with the values of the NVSHMEM
+// constants the reference passes through (get.cu:1181).
+enum cmp_op : int { CMP_EQ = 0, CMP_NE = 1, CMP_GT = 2, CMP_LE = 3, CMP_LT = 4,
+                    CMP_GE = 5 };
+
+// Kept for source compatibility (chare.h:41-49); the pending waits live in
+// the PE's state (rt::AsyncWait).
 struct async_wait_t {
   bool valid;
   uint64_t* ivars;
@@ -93,6 +98,37 @@ struct chare_proxy_base {
     const int big = map_rem * (map_per + 1);
     if (idx < big) return idx / (map_per + 1);
     return map_rem + (idx - big) / map_per;
+  }
+
+  // Single-threaded (chare.h:213-239): once every ivars[k], k < nelems,
+  // compares true (cmp) against cmp_value, the scheduler runs entry method
+  // ep of local chare idx once, with a null argument and no refnum check.
+  // ivars are device memory that other PEs (or the host) write; a writer
+  // that publishes data with its ivar releases before the store, and the
+  // scheduler acquires after the test succeeds.
+  __device__ void async_wait(uint64_t* ivars, size_t nelems, int cmp,
+                             uint64_t cmp_value, int idx, int ep) {
+    if (idx < start_idx || idx > end_idx || ep < 0 || ep >= em_count ||
+        cmp < CMP_EQ || cmp > CMP_GE) {
+      rt::raise_error(rt::ERR_BAD_CHARE, (uint32_t)idx);
+      return;
+    }
+    rt::PeState* ps = rt::pe_state();
+    for (int i = 0; i < ASYNC_WAIT_MAX; i++) {
+      rt::AsyncWait& w = ps->waits[i];
+      if (w.valid) continue;
+      w.ivars = ivars;
+      w.nelems = nelems;
+      w.cmp = cmp;
+      w.cmp_value = cmp_value;
+      w.proxy = id;
+      w.idx = idx;
+      w.ep = ep;
+      w.valid = 1;
+      ps->n_waits++;
+      return;
+    }
+    rt::raise_error(rt::ERR_WAITS_FULL, (uint32_t)ASYNC_WAIT_MAX);
   }
 
   // Refnum gate then entry method (chare.h:181-195).  Workgroup-collective.

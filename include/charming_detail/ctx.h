@@ -52,6 +52,7 @@ enum : uint32_t {
   ERR_SPIN_LIMIT = 8,
   ERR_BAD_CHARE = 9,
   ERR_UNSUPPORTED = 10,
+  ERR_WAITS_FULL = 11,
 };
 
 // control words: each on its own 128-byte line
@@ -93,8 +94,21 @@ struct Stats {
   unsigned long long pad;
 };
 
+// A pending async_wait (reference chare.h:41-49): when every ivars[k],
+// k < nelems, compares true against cmp_value, entry method ep of local
+// chare idx of proxy `proxy` runs once with a null argument.
+struct AsyncWait {
+  uint64_t* ivars;
+  uint64_t nelems;
+  uint64_t cmp_value;
+  int cmp, proxy, idx, ep;
+  int valid;
+};
+
 struct PeState {
   ProxyTable table;
+  int n_waits;         // valid entries of waits[]
+  AsyncWait waits[ASYNC_WAIT_MAX];
   uint64_t read_pos;   // logical position of the next inbox message
   uint64_t heap_used;  // bump allocator in the PE's heap slice
   uint32_t arena_used; // deferred arena bump pointer

@@ -44,7 +44,7 @@ static __constant__ Ctx g_ctx;
 
 constexpr uint64_t kFail = ~0ull;
 constexpr int kMaxPending = 16;
-enum : int { CMD_EXIT = 0, CMD_INBOX = 1, CMD_REVIVED = 2 };
+enum : int { CMD_EXIT = 0, CMD_INBOX = 1, CMD_REVIVED = 2, CMD_ASYNC = 3 };
 
 // Workgroup-shared scratch used to broadcast thread 0's decisions.  The
 // pending list holds publish words of messages already written but not yet
@@ -390,8 +390,26 @@ __device__ __noinline__ bool dispatch(envelope* env) {
   return true;
 }
 
+// Thread 0: does every ivar of the wait compare true?
+__device__ inline bool wait_satisfied(const AsyncWait& w) {
+  for (uint64_t k = 0; k < w.nelems; k++) {
+    const uint64_t v = ld_relaxed(w.ivars + k);
+    bool ok;
+    switch (w.cmp) {
+      case charm::CMP_EQ: ok = v == w.cmp_value; break;
+      case charm::CMP_NE: ok = v != w.cmp_value; break;
+      case charm::CMP_GT: ok = v > w.cmp_value; break;
+      case charm::CMP_LE: ok = v <= w.cmp_value; break;
+      case charm::CMP_LT: ok = v < w.cmp_value; break;
+      default: ok = v >= w.cmp_value; break;
+    }
+    if (!ok) return false;
+  }
+  return true;
+}
+
 // Thread 0: wait for work.  Revived messages go first (in arrival order),
-// then the inbox in FIFO order.
+// then the inbox in FIFO order, then satisfied async waits.
 __device__ int poll_next() {
   const Ctx& c = g_ctx;
   PeState* ps = pe_state();
@@ -422,6 +440,20 @@ __device__ int poll_next() {
       fence_acquire();
       L.ptr = ring + (ps->read_pos & (c.ring_bytes - 1));
       return CMD_INBOX;
+    }
+    if (ps->n_waits > 0) {
+      for (int i = 0; i < ASYNC_WAIT_MAX; i++) {
+        AsyncWait& w = ps->waits[i];
+        if (!w.valid || !wait_satisfied(w)) continue;
+        L.t_found = wall_clock64();
+        fence_acquire();
+        w.valid = 0;
+        ps->n_waits--;
+        L.arg = w.proxy;
+        L.n = w.idx;
+        L.pos = (uint64_t)w.ep;
+        return CMD_ASYNC;
+      }
     }
     if (ld_relaxed(&c.ctl[CTL_TERM]) != 0) return CMD_EXIT;
     if ((spins & 255) == 255) {
@@ -486,6 +518,25 @@ __device__ __noinline__ void process_revived() {
   }
 }
 
+// A satisfied async wait: its entry method, with a null argument.
+__device__ __noinline__ void process_async() {
+  Lds& L = lds();
+  PeState* ps = pe_state();
+  const int id = L.arg, idx = L.n, ep = (int)L.pos;
+  ProxyTable& t = ps->table;
+  if (id < 0 || id >= t.count || !t.proxies[id]) {
+    if (threadIdx.x == 0) raise_error(ERR_BAD_CHARE, (uint32_t)id);
+  } else {
+    t.proxies[id]->call(idx, ep, nullptr, -1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    flush_publish();
+    ps->stats.delivered++;
+    ps->stats.overhead_ticks += wall_clock64() - L.t_found;
+  }
+}
+
 __device__ __noinline__ void pe_loop() {
   Lds& L = lds();
   for (;;) {
@@ -495,8 +546,10 @@ __device__ __noinline__ void pe_loop() {
     if (cmd == CMD_EXIT) break;
     if (cmd == CMD_INBOX)
       process_inbox();
-    else
+    else if (cmd == CMD_REVIVED)
       process_revived();
+    else
+      process_async();
     __syncthreads();
   }
 }
@@ -547,6 +600,8 @@ __device__ void pe_begin() {
     ps->n_revived = 0;
     ps->ended = 0;
     ps->hw = 0;
+    ps->n_waits = 0;
+    for (int i = 0; i < ASYNC_WAIT_MAX; i++) ps->waits[i].valid = 0;
     lds().n_pend = 0;
     for (int i = 0; i < CHARMING_DEFERRED_MAX; i++) ps->deferred[i].state = 0;
     ps->stats = Stats{};

@@ -32,7 +32,7 @@ def test_jacobi_abi_version_without_gpu():
     assert charming_amd.jacobi_lib().chm_jacobi2d_abi_version() == 1
 
 
-@pytest.mark.parametrize("prog", ["hello", "refnum", "pingpong"])
+@pytest.mark.parametrize("prog", ["hello", "refnum", "pingpong", "async_wait"])
 def test_example_programs_built(prog):
     assert os.path.exists(os.path.join(ROOT, "charming_amd", "lib", f"libchm_{prog}.so"))
     assert os.path.exists(os.path.join(ROOT, "charming_amd", "bin", prog))

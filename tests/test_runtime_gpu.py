@@ -80,3 +80,26 @@ def test_pingpong_exact_payloads(capfd):
     assert m and int(m.group(1)) == 18 * 22 and int(m.group(2)) == 0
     assert "peer done: payload errors 0" in out
     assert int(st["deferred"]) == 0 and int(st["error"]) == 0
+
+
+@pytest.mark.parametrize("n_pes", [None, 3])
+def test_async_wait_on_device_ivars(capfd, n_pes):
+    R = 4
+    out, st = run(capfd, "async_wait", R, n_pes=n_pes)
+    P = int(st["pes"])
+    eq = re.findall(r"Round (\d+): all (\d+) ivars equal, payload errors (\d+)", out)
+    assert [(int(r), int(k), int(e)) for r, k, e in eq] == \
+        [(r, P - 1, 0) for r in range(1, R + 1)]
+    ge = re.findall(r"Round (\d+): counter reached (\d+)", out)
+    assert [(int(r), int(c)) for r, c in ge] == \
+        [(r, r * (P - 1)) for r in range(1, R + 1)]
+    assert f"Async wait done: {R} rounds, errors 0" in out
+    # init + one publish per other chare per round + two fired waits per round
+    assert int(st["delivered"]) == 1 + (P - 1) * R + 2 * R
+    assert int(st["error"]) == 0
+
+
+def test_async_wait_single_pe(capfd):
+    out, st = run(capfd, "async_wait", 3, n_pes=1)
+    assert "Async wait done: nothing to wait for" in out
+    assert int(st["error"]) == 0
