@@ -59,7 +59,8 @@ class Result(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("panel_chunks", C.c_int), ("rows_in_flight", C.c_int),
                 ("band_rows", C.c_int), ("wg_waves", C.c_int),
-                ("overlap", C.c_int), ("depth", C.c_int)]
+                ("overlap", C.c_int), ("depth", C.c_int),
+                ("halo_copy", C.c_int)]
 
 
 class BatchProblem(C.Structure):
@@ -166,9 +167,9 @@ class Jacobi2D:
         self.h = h
 
     def set_tuning(self, panel_chunks=0, rows_in_flight=0, band_rows=0,
-                   wg_waves=0, overlap=0, depth=0):
+                   wg_waves=0, overlap=0, depth=0, halo_copy=0):
         t = Tuning(panel_chunks, rows_in_flight, band_rows, wg_waves, overlap,
-                   depth)
+                   depth, halo_copy)
         _check(jacobi_lib().chm_jacobi2d_set_tuning(self.h, C.byref(t)))
 
     def reset(self):

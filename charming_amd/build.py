@@ -93,6 +93,46 @@ def _link_exe(name, srcs, verbose):
         raise RuntimeError(f"link failed for {name}:\n{p.stdout}{p.stderr}")
 
 
+# Per-lane stack the runtime gives the scheduler kernel (chm_config
+# .stack_bytes default); chare programs' entry-method chains must fit in it.
+STACK_BYTES = 4096
+# chare programs (translation units that include <charming.h>)
+CHARE_PROGRAMS = ["jacobi2d/persistent.hip", "examples/hello.hip",
+                  "examples/refnum.hip", "examples/pingpong.hip",
+                  "examples/async_wait.hip"]
+
+
+def stack_need(src):
+    """Bytes of per-lane stack the deepest call chain of a chare program's
+    scheduler kernel needs: the kernel's own chain down to the dispatch of
+    an entry method, plus the largest entry method (reached through a
+    function pointer, so the compiler gives the kernel a dynamic stack and
+    cannot size it).  Computed from the frame sizes in the device assembly."""
+    import re
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        out = os.path.join(d, "k.s")
+        cmd = [HIPCC] + DEVICE_FLAGS + ["-S", "--cuda-device-only",
+                                        os.path.join(CSRC, src), "-o", out]
+        p = subprocess.run(cmd, capture_output=True, text=True)
+        if p.returncode != 0:
+            raise RuntimeError(f"hipcc -S failed on {src}:\n{p.stderr}")
+        asm = open(out).read()
+    defs = dict(re.findall(r"\.set (\S+)\.private_seg_size, (.+)", asm))
+    memo = {}
+
+    def size(name):
+        if name not in memo:
+            expr = re.sub(r"([.\w$]+?)\.private_seg_size",
+                          lambda m: str(size(m.group(1))), defs.get(name, "0"))
+            memo[name] = eval(expr.replace("max(", "max(0,"),
+                              {"__builtins__": {}, "max": max})
+        return memo[name]
+    kernel = max(size(n) for n in defs if "scheduler_kernel" in n)
+    entries = [size(n) for n in defs if "entry_method" in n]
+    return kernel + max(entries, default=0)
+
+
 def build(verbose=False, jobs=None):
     os.makedirs(LIBDIR, exist_ok=True)
     os.makedirs(OBJDIR, exist_ok=True)

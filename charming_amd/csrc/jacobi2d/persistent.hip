@@ -60,6 +60,8 @@ struct JacobiDesc {
   int overlap;            // sweep edges first, send, then the interior
   uint64_t* recv_ticks;   // per chare: ticks inside recv_ghosts (unpack)
   int depth;              // halo depth = sweeps per exchange (1 or 2)
+  int halo_copy;          // 1: halos staged and copied by invoke (else the
+                          // sweep writes them into the messages in place)
 };
 
 // Chare arrays always hold halos two deep: rows -1 .. bh+2 of
@@ -114,6 +116,8 @@ struct Block : charm::chare {
 
   __device__ Block() {}
   __device__ void init(void* arg);
+  __device__ size_t halo_bytes(int k) const;
+  __device__ void fill_corners(double* const* bufs);
   __device__ void send_boundaries();
   __device__ void recv_ghosts(void* arg);
   __device__ void update();
@@ -377,28 +381,39 @@ __device__ void Block::init(void* arg) {
   if (neighbor_count == 0) update();
 }
 
+// Bytes of the halo message toward neighbour k: the dir slot, then `depth`
+// rows/columns of the edge, or one corner cell.
+__device__ size_t Block::halo_bytes(int k) const {
+  size_t len = 1;
+  if (k == LEFT || k == RIGHT) len = (size_t)depth * block_height;
+  if (k == TOP || k == BOTTOM) len = (size_t)depth * block_width;
+  return sizeof(double) + sizeof(double) * len;
+}
+
+// Thread 0: with two-deep halos the four corner cells go to the diagonal
+// neighbours, taken from the top and bottom edges already written (a
+// diagonal neighbour implies both).  bufs[k] = [dir slot | payload].
+__device__ void Block::fill_corners(double* const* bufs) {
+  if (depth != 2) return;
+  const int bw = block_width;
+  const double* top = bufs[TOP] + 1;
+  const double* bot = bufs[BOTTOM] + 1 + bw;
+  if (neighbor_index[TOP_LEFT] >= 0) bufs[TOP_LEFT][1] = top[0];
+  if (neighbor_index[TOP_RIGHT] >= 0) bufs[TOP_RIGHT][1] = top[bw - 1];
+  if (neighbor_index[BOTTOM_LEFT] >= 0) bufs[BOTTOM_LEFT][1] = bot[0];
+  if (neighbor_index[BOTTOM_RIGHT] >= 0) bufs[BOTTOM_RIGHT][1] = bot[bw - 1];
+}
+
 __device__ void Block::send_boundaries() {
   // (jacobi2d.cu:152-194) sends tagged with the iteration: four edges
-  // `depth` deep, and with two-deep halos the four corner cells, taken from
-  // the top and bottom edge staging (a diagonal neighbour implies both)
-  const int it = iter, bw = block_width, D = depth;
-  if (D == 2 && threadIdx.x == 0) {
-    const double* top = boundaries[TOP] + 1;
-    const double* bot = boundaries[BOTTOM] + 1 + (D - 1) * bw;
-    if (neighbor_index[TOP_LEFT] >= 0) boundaries[TOP_LEFT][1] = top[0];
-    if (neighbor_index[TOP_RIGHT] >= 0) boundaries[TOP_RIGHT][1] = top[bw - 1];
-    if (neighbor_index[BOTTOM_LEFT] >= 0) boundaries[BOTTOM_LEFT][1] = bot[0];
-    if (neighbor_index[BOTTOM_RIGHT] >= 0)
-      boundaries[BOTTOM_RIGHT][1] = bot[bw - 1];
-  }
+  // `depth` deep, and with two-deep halos the four corner cells, copied
+  // from the staging buffers
+  if (threadIdx.x == 0) fill_corners(boundaries);
   __syncthreads();
   for (int k = 0; k < N_NEIGHBORS; k++) {
     if (neighbor_index[k] < 0) continue;
-    size_t len = 1;
-    if (k == LEFT || k == RIGHT) len = (size_t)D * block_height;
-    if (k == TOP || k == BOTTOM) len = (size_t)D * block_width;
-    block_proxy->invoke(neighbor_index[k], 1, boundaries[k],
-                        sizeof(double) + sizeof(double) * len, it);
+    block_proxy->invoke(neighbor_index[k], 1, boundaries[k], halo_bytes(k),
+                        iter);
   }
 }
 
@@ -473,14 +488,41 @@ __device__ void Block::update() {
     const bool last = (iter + steps == n_iters);
     double* t = grids[cur];
     double* nt = grids[cur ^ 1];
-    EdgeOut eo;
-    eo.top = (neighbor_index[TOP] >= 0) ? boundaries[TOP] + 1 : nullptr;
-    eo.bottom = (neighbor_index[BOTTOM] >= 0) ? boundaries[BOTTOM] + 1 : nullptr;
-    eo.left = (neighbor_index[LEFT] >= 0) ? boundaries[LEFT] + 1 : nullptr;
-    eo.right = (neighbor_index[RIGHT] >= 0) ? boundaries[RIGHT] + 1 : nullptr;
     // overlap: phase 0 = everything that feeds the outgoing halos, sent
     // before phase 1 sweeps the interior; otherwise one whole-block sweep
     const bool ov = g_jdesc.overlap != 0 && depth == 1;
+    // zero-copy halos: the outgoing messages are allocated in the
+    // neighbours' inboxes first and the sweep writes the edges straight
+    // into them (no staging copy); sent after the sweep
+    const bool zc = !ov && !last && g_jdesc.halo_copy == 0;
+    // (kept in LDS: the values are the workgroup's, and the stack of an
+    // entry method is small)
+    __shared__ charm::envelope* s_env[N_NEIGHBORS];
+    __shared__ size_t s_off[N_NEIGHBORS];
+    __shared__ double* s_dst[N_NEIGHBORS];
+    for (int k = 0; k < N_NEIGHBORS; k++) {
+      double* d = (neighbor_index[k] >= 0) ? boundaries[k] : nullptr;
+      if (zc && neighbor_index[k] >= 0) {
+        charm::message m;
+        d = (double*)block_proxy->alloc(m, neighbor_index[k], 1, halo_bytes(k),
+                                        iter + steps);
+        if (threadIdx.x == 0) {
+          s_env[k] = m.env;
+          s_off[k] = m.offset;
+          if (d) {
+            ((int*)d)[0] = opposite_side(k);
+            ((int*)d)[1] = 0;
+          }
+        }
+      }
+      if (threadIdx.x == 0) s_dst[k] = d;
+    }
+    __syncthreads();
+    EdgeOut eo;
+    eo.top = s_dst[TOP] ? s_dst[TOP] + 1 : nullptr;
+    eo.bottom = s_dst[BOTTOM] ? s_dst[BOTTOM] + 1 : nullptr;
+    eo.left = s_dst[LEFT] ? s_dst[LEFT] + 1 : nullptr;
+    eo.right = s_dst[RIGHT] ? s_dst[RIGHT] + 1 : nullptr;
     if (steps == 2) {
       EdgeOut2 eo2 = {eo.top, eo.bottom, eo.left, eo.right};
       const RingSides rs = {neighbor_index[TOP] < 0, neighbor_index[BOTTOM] < 0,
@@ -505,7 +547,17 @@ __device__ void Block::update() {
     }
     __syncthreads();
     // the halos of u_{iter} leave before the interior is swept
-    if (!done) {
+    if (!done && zc) {
+      if (threadIdx.x == 0) fill_corners(s_dst);
+      for (int k = 0; k < N_NEIGHBORS; k++) {
+        if (neighbor_index[k] < 0) continue;
+        charm::message m;
+        m.env = s_env[k];
+        m.offset = s_off[k];
+        m.dst_pe = block_proxy->pe_of(neighbor_index[k]);
+        block_proxy->send(m);
+      }
+    } else if (!done) {
       send_boundaries();
       if (ov) charm::flush_sends();
     }
@@ -720,6 +772,7 @@ int persistent_solve(Handle* h, chm_jacobi_result* res) {
   d.tune_nc = h->tune.panel_chunks;
   d.tune_u = h->tune.rows_in_flight;
   d.overlap = h->tune.overlap;
+  d.halo_copy = h->tune.halo_copy;
   // two-deep halos need every chare at least 2 x 2
   d.depth = h->tune.depth ? h->tune.depth : (ps->min_side >= 2 ? 2 : 1);
   if (d.depth != 1 && d.depth != 2)

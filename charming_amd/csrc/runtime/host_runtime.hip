@@ -106,6 +106,7 @@ void default_config(chm_config* c) {
   c->max_msg_bytes = (1ull << 20) + 64;
   c->deferred_bytes = 2ull << 20;
   c->heap_bytes = 1ull << 20;
+  c->stack_bytes = 4096;
 }
 
 // Environment overrides for programs started through the library main.
@@ -114,6 +115,7 @@ void env_config(chm_config* c) {
   if (const char* s = getenv("CHM_TIMEOUT_S")) c->timeout_s = atof(s);
   if (const char* s = getenv("CHM_RING_BYTES")) c->ring_bytes = strtoull(s, 0, 0);
   if (const char* s = getenv("CHM_VERBOSE")) c->verbose = atoi(s);
+  if (const char* s = getenv("CHM_STACK_BYTES")) c->stack_bytes = strtoull(s, 0, 0);
 }
 
 }  // namespace
@@ -161,6 +163,7 @@ int chm_init(int argc, char** argv, const chm_config* cfg_in) {
     cfg.timeout_s = cfg_in->timeout_s;
     cfg.spin_limit = cfg_in->spin_limit;
     cfg.verbose = cfg_in->verbose;
+    if (cfg_in->stack_bytes) cfg.stack_bytes = cfg_in->stack_bytes;
   } else {
     env_config(&cfg);
   }
@@ -170,6 +173,10 @@ int chm_init(int argc, char** argv, const chm_config* cfg_in) {
   if (cfg.device < 0 || cfg.device >= ndev)
     return chm::fail(CHM_E_INVALID, "device %d out of range", cfg.device);
   CHM_TRY_HIP(hipSetDevice(cfg.device));
+  // the scheduler kernel's stack is dynamic (entry methods are indirect
+  // calls): its per-lane size comes from this limit, not from the compiler
+  if (cfg.stack_bytes < 1024) cfg.stack_bytes = 1024;
+  CHM_TRY_HIP(hipDeviceSetLimit(hipLimitStackSize, cfg.stack_bytes));
   hipDeviceProp_t prop;
   CHM_TRY_HIP(hipGetDeviceProperties(&prop, cfg.device));
   if (!prop.cooperativeLaunch)

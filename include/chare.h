@@ -238,6 +238,15 @@ struct chare_proxy : chare_proxy_base {
     for (int i = 0; i < n_total; i++) invoke(i, ep, buf, size);
   }
 
+  // Zero-copy send (workgroup-collective): allocate a message for entry
+  // method ep of chare idx in its PE's inbox, fill the returned payload in
+  // place, then send(msg).  No waits on other PEs in between.
+  inline __device__ void* alloc(message& msg, int idx, int ep, size_t size,
+                                int refnum = -1) {
+    return alloc_user_msg(msg, id, idx, ep, size, pe_of(idx), refnum);
+  }
+  inline __device__ void send(message& msg) { send_user_msg(msg); }
+
   // Single-threaded (chare.h:243-253): set the refnum of a local chare and
   // revive its deferred messages carrying that refnum.
   inline __device__ void set_refnum(int idx, int val) {

@@ -35,6 +35,11 @@ typedef struct chm_config {
   double timeout_s;         /* watchdog: stop the kernel after this long (0 = none) */
   unsigned long long spin_limit; /* polls before a blocked send fails (0 = none) */
   int verbose;              /* print the launch configuration (charming.cu:214-227) */
+  size_t stack_bytes;       /* per-lane stack of the scheduler kernel (0 = 4096).
+                               Entry methods are called through pointers, so the
+                               compiler cannot size the stack of the chain below
+                               them; the runtime's default of 1 KiB is too small
+                               for some entry methods */
 } chm_config;
 
 typedef struct chm_stats {

@@ -232,9 +232,24 @@ __device__ uint64_t reserve(int dst, uint64_t slot) {
   return pos;
 }
 
-// Workgroup-collective send of one message to PE dst.
-__device__ void send_msg(int dst, msgtype type, int chare_id, int chare_idx,
-                         int ep, int refnum, const void* buf, size_t payload) {
+// Thread 0: queue the publication of the message at ring position pos of
+// dst (released with the others by flush_publish).
+__device__ void queue_publish(int dst, uint64_t pos) {
+  const Ctx& c = g_ctx;
+  Lds& L = lds();
+  const uint64_t g = pos / kGranule;
+  if (L.n_pend == kMaxPending) flush_publish();
+  L.pend_addr[L.n_pend] =
+      &c.ready[(uint64_t)dst * n_granules() + (g & (n_granules() - 1))];
+  L.pend_tag[L.n_pend] = g + 1;
+  L.n_pend++;
+  pe_state()->stats.sent++;
+}
+
+// Workgroup-collective: reserve a message in dst's ring and write its
+// headers; returns its address (nullptr on failure) and ring position.
+__device__ char* alloc_msg(int dst, msgtype type, int chare_id, int chare_idx,
+                           int ep, int refnum, size_t payload, uint64_t* pos_out) {
   const Ctx& c = g_ctx;
   Lds& L = lds();
   const uint64_t alloc = envelope::alloc_size(type, payload);
@@ -243,7 +258,8 @@ __device__ void send_msg(int dst, msgtype type, int chare_id, int chare_idx,
   if (threadIdx.x == 0) L.pos = reserve(dst, slot);
   __syncthreads();
   const uint64_t pos = L.pos;
-  if (pos == kFail) return;
+  *pos_out = pos;
+  if (pos == kFail) return nullptr;
   char* base = c.rings + (uint64_t)dst * c.ring_stride +
                (pos & (c.ring_bytes - 1));
   if (threadIdx.x == 0) {
@@ -251,19 +267,21 @@ __device__ void send_msg(int dst, msgtype type, int chare_id, int chare_idx,
     new (base + sizeof(envelope))
         regular_msg(chare_id, chare_idx, ep, refnum);
   }
+  return base;
+}
+
+// Workgroup-collective send of one message to PE dst.
+__device__ void send_msg(int dst, msgtype type, int chare_id, int chare_idx,
+                         int ep, int refnum, const void* buf, size_t payload) {
+  uint64_t pos = 0;
+  char* base = alloc_msg(dst, type, chare_id, chare_idx, ep, refnum, payload,
+                         &pos);
+  if (!base) return;
   if (payload) wg_copy(base + sizeof(envelope) + sizeof(regular_msg),
                        (const char*)buf, payload);
   wave_drain();
   __syncthreads();
-  if (threadIdx.x == 0) {
-    const uint64_t g = pos / kGranule;
-    if (L.n_pend == kMaxPending) flush_publish();
-    L.pend_addr[L.n_pend] =
-        &c.ready[(uint64_t)dst * n_granules() + (g & (n_granules() - 1))];
-    L.pend_tag[L.n_pend] = g + 1;
-    L.n_pend++;
-    pe_state()->stats.sent++;
-  }
+  if (threadIdx.x == 0) queue_publish(dst, pos);
 }
 
 // ---------------------------------------------------------------------------
@@ -618,6 +636,26 @@ __device__ void send_reg_msg(int chare_id, int chare_idx, int ep_id, void* buf,
                              size_t payload_size, int dst_pe, int refnum) {
   rt::send_msg(dst_pe, msgtype::regular, chare_id, chare_idx, ep_id, refnum,
                buf, payload_size);
+}
+
+__device__ void* alloc_user_msg(message& msg, int chare_id, int chare_idx,
+                                int ep_id, size_t payload_size, int dst_pe,
+                                int refnum) {
+  uint64_t pos = 0;
+  char* base = rt::alloc_msg(dst_pe, msgtype::user, chare_id, chare_idx, ep_id,
+                             refnum, payload_size, &pos);
+  msg.env = (envelope*)base;
+  msg.offset = pos;
+  msg.dst_pe = dst_pe;
+  return msg.payload();
+}
+
+__device__ void send_user_msg(message& msg) {
+  if (!msg.env) return;
+  rt::wave_drain();
+  __syncthreads();
+  if (threadIdx.x == 0) rt::queue_publish(msg.dst_pe, msg.offset);
+  msg.env = nullptr;
 }
 
 // Single-threaded (get.cu:1316-1345): deferred messages of (chare, refnum)

@@ -90,14 +90,23 @@ struct alignas(ALIGN_SIZE) envelope {
 static_assert(sizeof(envelope) == 16, "envelope must stay 16 bytes");
 static_assert(sizeof(regular_msg) == 16, "regular_msg must stay 16 bytes");
 
-// User message handle (message.h:85-95); the user-message API is not part
-// of the reference either (commented out there), so only the type exists.
+// User message handle (message.h:85-95).  The reference sketches a
+// user-message API and leaves it commented out (message.h:84-95, 113-118,
+// get.cu:1362-1398); here it is the zero-copy send: the message is
+// allocated directly in the destination PE's inbox ring, the workgroup
+// writes the payload in place, and send_user_msg publishes it.  Between
+// alloc and send the destination consumes nothing behind this message, so
+// the sender must not wait on other PEs in between.
 struct alignas(ALIGN_SIZE) message {
-  envelope* env;
-  size_t offset;
+  envelope* env;  // the message in the destination ring (nullptr: none)
+  size_t offset;  // its position in that ring
   int dst_pe;
 
   __device__ message() : env(nullptr), offset(0), dst_pe(-1) {}
+  // payload of the allocated message
+  __device__ void* payload() const {
+    return env ? (char*)env + sizeof(envelope) + sizeof(regular_msg) : nullptr;
+  }
 };
 
 // Send API (message.h:97-114).  All are workgroup-collective: every thread
@@ -105,6 +114,15 @@ struct alignas(ALIGN_SIZE) message {
 __device__ void send_reg_msg(int chare_id, int chare_idx, int ep_id, void* buf,
                              size_t payload_size, int dst_pe, int refnum);
 __device__ void send_begin_term_msg();
+// Zero-copy send (user messages): reserve `payload_size` bytes for entry
+// method ep_id of chare chare_idx (proxy chare_id) in dst_pe's inbox and
+// write the headers; returns the payload pointer (nullptr when the runtime
+// is stopping).  The workgroup fills the payload, then send_user_msg
+// publishes it (batched with the entry method's other sends).
+__device__ void* alloc_user_msg(message& msg, int chare_id, int chare_idx,
+                                int ep_id, size_t payload_size, int dst_pe,
+                                int refnum);
+__device__ void send_user_msg(message& msg);
 __device__ void revive_mismatches(int chare_id, int chare_idx, int refnum);
 
 }  // namespace charm

@@ -88,15 +88,18 @@ def test_default_decomposition_uses_all_pes(cam):
     assert r.residual == res
 
 
+@pytest.mark.parametrize("halo_copy", [0, 1])
 @pytest.mark.parametrize("depth", [1, 2])
 @pytest.mark.parametrize("W,H,cr,cc,iters", [
     (300, 200, 3, 5, 17), (257, 129, 4, 3, 10), (64, 64, 2, 2, 1),
     (100, 100, 5, 5, 2), (130, 70, 1, 1, 9), (500, 40, 2, 7, 5)])
-def test_halo_depths_bit_exact(cam, depth, W, H, cr, cc, iters):
+def test_halo_depths_bit_exact(cam, depth, halo_copy, W, H, cr, cc, iters):
+    # depth 1/2 = sweeps per exchange; halo_copy 0 = zero-copy halo messages
+    # (allocated in the neighbours' inboxes, written by the sweep), 1 = staged
     ref, res = O.jacobi_run(W, H, iters, seed=W + H)
     with cam.Jacobi2D(W, H, iters, form="persistent", seed=W + H,
                       chares=(cr, cc)) as j:
-        j.set_tuning(depth=depth)
+        j.set_tuning(depth=depth, halo_copy=halo_copy)
         r = j.solve()
         assert np.array_equal(j.grid(), ref)
         assert r.residual == res and r.hash == O.grid_hash(ref)
