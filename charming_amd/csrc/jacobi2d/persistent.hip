@@ -287,12 +287,15 @@ __device__ __noinline__ void chare_sweep2(const double* in, double* out,
 __device__ void sweep2_dispatch(const double* in, double* out, int64_t P,
                                 int bw, int bh, bool res, bool pack,
                                 const EdgeOut2& eo, RingSides rs) {
+  // default 1 chunk x 1 row in flight: the fused sweep keeps two rings of
+  // rows in registers, and with the 128-VGPR cap of entry methods deeper
+  // shapes spill (measured on 16384^2: 1x1 517, 1x2 513, 1x4 473 Gcell/s)
   int nc = g_jdesc.tune_nc, u = g_jdesc.tune_u;
   if (nc == 0) {
     nc = 1;
-    u = 4;
+    u = 1;
   }
-  if (u == 0) u = 2;
+  if (u == 0) u = 1;
 #define CHM_CASE2(NC_, U_)                                                   \
   if (nc == NC_ && u == U_) {                                                \
     if (res)                                                                 \
@@ -303,14 +306,15 @@ __device__ void sweep2_dispatch(const double* in, double* out, int64_t P,
       chare_sweep2<NC_, U_, false, false>(in, out, P, bw, bh, eo, rs);       \
     return;                                                                  \
   }
-  CHM_CASE2(1, 2) CHM_CASE2(1, 4) CHM_CASE2(2, 2)
+  CHM_CASE2(1, 1) CHM_CASE2(1, 2) CHM_CASE2(1, 4) CHM_CASE2(2, 1)
+  CHM_CASE2(2, 2)
 #undef CHM_CASE2
   if (res)
-    chare_sweep2<1, 4, true, false>(in, out, P, bw, bh, eo, rs);
+    chare_sweep2<1, 1, true, false>(in, out, P, bw, bh, eo, rs);
   else if (pack)
-    chare_sweep2<1, 4, false, true>(in, out, P, bw, bh, eo, rs);
+    chare_sweep2<1, 1, false, true>(in, out, P, bw, bh, eo, rs);
   else
-    chare_sweep2<1, 4, false, false>(in, out, P, bw, bh, eo, rs);
+    chare_sweep2<1, 1, false, false>(in, out, P, bw, bh, eo, rs);
 }
 
 __device__ void Block::init(void* arg) {
@@ -779,6 +783,16 @@ int persistent_solve(Handle* h, chm_jacobi_result* res) {
     return fail(CHM_E_INVALID, "halo depth %d (1 or 2)", d.depth);
   if (d.depth == 2 && ps->min_side < 2)
     return fail(CHM_E_INVALID, "halo depth 2 needs chares of at least 2x2");
+  if (d.depth == 2 && d.tune_nc) {
+    // shapes of the fused two-sweep pass (sweep2_dispatch)
+    const int nc = d.tune_nc, u = d.tune_u ? d.tune_u : 1;
+    const bool ok = (nc == 1 && (u == 1 || u == 2 || u == 4)) ||
+                    (nc == 2 && (u == 1 || u == 2));
+    if (!ok)
+      return fail(CHM_E_INVALID,
+                  "halo depth 2 sweeps with panel_chunks x rows_in_flight of "
+                  "1x1, 1x2, 1x4, 2x1 or 2x2 (asked %dx%d)", nc, u);
+  }
   CHM_TRY(upload_desc(d));
   CHM_TRY_HIP(hipMemset(d.res_acc, 0, 4 * sizeof(uint64_t)));
   CHM_TRY_HIP(hipMemset(d.recv_ticks, 0, (size_t)ps->n_chares * 8));

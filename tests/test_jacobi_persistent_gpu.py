@@ -63,13 +63,14 @@ def test_reference_init_kat(cam):
     assert ref1[1, 1] == 4.0 and ref1[1, 2] == 2.0
 
 
-@pytest.mark.parametrize("nc,u", [(1, 4), (2, 2), (4, 1), (1, 1)])
-def test_tunings_and_rerun(cam, nc, u):
+@pytest.mark.parametrize("nc,u,depth", [(1, 4, 0), (2, 2, 0), (4, 1, 1), (1, 1, 0),
+                                        (2, 1, 0), (1, 2, 1)])
+def test_tunings_and_rerun(cam, nc, u, depth):
     W, H, iters = 1100, 700, 8
     ref, res = O.jacobi_run(W, H, iters, seed=3)
     with cam.Jacobi2D(W, H, iters, form="persistent", seed=3,
                       chares=(3, 2)) as j:
-        j.set_tuning(nc, u)
+        j.set_tuning(nc, u, depth=depth)
         r = j.solve()
         assert np.array_equal(j.grid(), ref) and r.residual == res
         j.reset()
@@ -115,6 +116,13 @@ def test_fused_sweep_shapes(cam, nc, u):
         j.set_tuning(nc, u, depth=2)
         r = j.solve()
         assert np.array_equal(j.grid(), ref) and r.residual == res
+
+
+def test_unsupported_fused_shape_is_refused(cam):
+    with cam.Jacobi2D(256, 256, 4, form="persistent", seed=1, chares=(2, 2)) as j:
+        j.set_tuning(4, 1, depth=2)
+        with pytest.raises(cam.ChmError):
+            j.solve()
 
 
 def test_thin_chares_fall_back_to_depth_one(cam):
