@@ -174,6 +174,12 @@ def main():
     ap.add_argument("--serve-rates", default="10,100,2000")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--depth", type=int, default=0,
+                    help="halo depth of the persistent form = sweeps fused "
+                         "per exchange (0 = the library default)")
+    ap.add_argument("--shape", default="0,0",
+                    help="fused-sweep shape: panel chunks,rows in flight "
+                         "(0,0 = the library default)")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run the streaming copy of a known size (1 GiB "
                          "read + 1 GiB written per launch) used to scale "
@@ -197,6 +203,9 @@ def main():
         return
 
     j = cam.Jacobi2D(W, H, ITERS, form="persistent", seed=SEED, chares=CHARES)
+    nc, u = (int(x) for x in args.shape.split(","))
+    if args.depth or nc or u:
+        j.set_tuning(nc, u, depth=args.depth)
     for _ in range(args.warmup):
         j.solve()
     kern_ms, step_ms = [], []
@@ -235,7 +244,9 @@ def main():
                                f"chares, {ITERS} iterations, persistent "
                                "message-driven",
                    "grid": [W, H], "chares": list(CHARES), "iters": ITERS,
-                   "form": "persistent", "parallelism": "1 GPU"},
+                   "form": "persistent", "parallelism": "1 GPU",
+                   "halo_depth": args.depth or "default",
+                   "sweep_shape": args.shape},
         "hbm_gbs": round(BYTES_PER_CELL * value, 1),
         # `achieved` counts the algorithmic 16 B per cell-update; the chares
         # sweep twice per pass (two-deep halos), so the bytes really moved

@@ -18,11 +18,12 @@
 //     is harmless only for square chare grids);
 //   - the last row/column of chares takes the remainder of an uneven split;
 //   - a chare with no neighbours iterates without waiting;
-//   - temporal blocking (halo depth 2, the default): chares exchange halos
-//     two cells deep (four edges of two rows/columns plus the four corner
-//     cells from the diagonal neighbours) once per two sweeps, and sweep
-//     twice in one pass over their block with the intermediate grid held in
-//     registers; the arithmetic, and so every grid, is unchanged.
+//   - temporal blocking (halo depth D = 2, the default, or 3): chares
+//     exchange halos D cells deep (four edges of D rows/columns plus four
+//     (D-1) x (D-1) corner blocks from the diagonal neighbours) once per D
+//     sweeps, and sweep D times in one pass over their block with the
+//     intermediate grids held in registers; the arithmetic, and so every
+//     grid, is unchanged.
 #define CHARMING_PE_THREADS 1024
 #include <charming.h>
 
@@ -59,14 +60,14 @@ struct JacobiDesc {
   int tune_nc, tune_u;
   int overlap;            // sweep edges first, send, then the interior
   uint64_t* recv_ticks;   // per chare: ticks inside recv_ghosts (unpack)
-  int depth;              // halo depth = sweeps per exchange (1 or 2)
+  int depth;              // halo depth = sweeps per exchange (1, 2 or 3)
   int halo_copy;          // 1: halos staged and copied by invoke (else the
                           // sweep writes them into the messages in place)
 };
 
-// Chare arrays always hold halos two deep: rows -1 .. bh+2 of
-// pitch_for_depth(bw, 2) doubles; `base` points at row 0.
-constexpr int kHaloRows = 2;
+// Chare arrays always hold halos three deep: rows -2 .. bh+3 of
+// pitch_for_depth(bw, 3) doubles; grids[] point at row 0.
+constexpr int kHaloRows = 3;
 __host__ __device__ inline int64_t chare_pitch(int bw) {
   return pitch_for_depth(bw, kHaloRows);
 }
@@ -284,6 +285,88 @@ __device__ __noinline__ void chare_sweep2(const double* in, double* out,
   }
 }
 
+// Values the whole wave shares, moved into scalar registers: an entry
+// method's callees receive their arguments in vector registers, which the
+// deeper fused sweeps cannot spare.
+__device__ inline int wuni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline int64_t wuni(int64_t v) {
+  const uint64_t u = (uint64_t)v;
+  return (int64_t)(((uint64_t)(uint32_t)wuni((int)(u >> 32)) << 32) |
+                   (uint32_t)wuni((int)(uint32_t)u));
+}
+template <class T>
+__device__ inline T* wuni(T* p) {
+  return (T*)wuni((int64_t)p);
+}
+
+// Three fused sweeps of the whole block (halos three deep), panels x bands.
+template <int NC, int U, bool RES, bool PACK>
+__device__ __noinline__ void chare_sweep3(const double* in, double* out,
+                                          int64_t P, int bw, int bh,
+                                          EdgeOut2 eo, RingSides rs) {
+  in = wuni(in);
+  out = wuni(out);
+  P = wuni(P);
+  bw = wuni(bw);
+  bh = wuni(bh);
+  eo.top = wuni(eo.top);
+  eo.bottom = wuni(eo.bottom);
+  eo.left = wuni(eo.left);
+  eo.right = wuni(eo.right);
+  rs.top = wuni((int)rs.top);
+  rs.bottom = wuni((int)rs.bottom);
+  rs.left = wuni((int)rs.left);
+  rs.right = wuni((int)rs.right);
+  const int wave = threadIdx.x / 64, WV = blockDim.x / 64;
+  const int pw = 128 * NC;
+  const int S = (bw + pw - 1) / pw;
+  int nb = WV / S;
+  if (nb < 1) nb = 1;
+  if (nb > bh) nb = bh;
+  const int br = (bh + nb - 1) / nb;
+  u128acc acc = {0, 0};
+  for (int t = wave; t < S * nb; t += WV) {
+    const int panel = t % S, b = t / S;
+    const int r0 = 1 + b * br, r1 = min(r0 + br, bh + 1);
+    sweep_panel3<NC, U, RES, PACK>(in, out, P, bw, bh, 1 + panel * pw, r0, r1,
+                                   acc, eo, rs);
+  }
+  if (RES) {
+    acc = wave_reduce_u128(acc);
+    if (__lane_id() == 0) atomic_add_u128(g_jdesc.res_acc, acc);
+  }
+}
+
+__device__ void sweep3_dispatch(const double* in, double* out, int64_t P,
+                                int bw, int bh, bool res, bool pack,
+                                const EdgeOut2& eo, RingSides rs) {
+  // 1 chunk x 1 row in flight (three rings of rows live in registers)
+  int nc = g_jdesc.tune_nc, u = g_jdesc.tune_u;
+  if (nc == 0) {
+    nc = 1;
+    u = 1;
+  }
+  if (u == 0) u = 1;
+#define CHM_CASE3(NC_, U_)                                                   \
+  if (nc == NC_ && u == U_) {                                                \
+    if (res)                                                                 \
+      chare_sweep3<NC_, U_, true, false>(in, out, P, bw, bh, eo, rs);        \
+    else if (pack)                                                           \
+      chare_sweep3<NC_, U_, false, true>(in, out, P, bw, bh, eo, rs);        \
+    else                                                                     \
+      chare_sweep3<NC_, U_, false, false>(in, out, P, bw, bh, eo, rs);       \
+    return;                                                                  \
+  }
+  CHM_CASE3(1, 1) CHM_CASE3(1, 2)
+#undef CHM_CASE3
+  if (res)
+    chare_sweep3<1, 1, true, false>(in, out, P, bw, bh, eo, rs);
+  else if (pack)
+    chare_sweep3<1, 1, false, true>(in, out, P, bw, bh, eo, rs);
+  else
+    chare_sweep3<1, 1, false, false>(in, out, P, bw, bh, eo, rs);
+}
+
 __device__ void sweep2_dispatch(const double* in, double* out, int64_t P,
                                 int bw, int bh, bool res, bool pack,
                                 const EdgeOut2& eo, RingSides rs) {
@@ -337,8 +420,8 @@ __device__ void Block::init(void* arg) {
     neighbor_index[RIGHT] = (col == cc - 1) ? -1 : index + 1;
     neighbor_index[TOP] = (row == 0) ? -1 : index - cc;
     neighbor_index[BOTTOM] = (row == cr - 1) ? -1 : index + cc;
-    // diagonal neighbours supply the corner cells of two-deep halos
-    const bool dg = depth == 2;
+    // diagonal neighbours supply the corner blocks of deeper halos
+    const bool dg = depth >= 2;
     neighbor_index[TOP_LEFT] = (dg && row > 0 && col > 0) ? index - cc - 1 : -1;
     neighbor_index[TOP_RIGHT] =
         (dg && row > 0 && col < cc - 1) ? index - cc + 1 : -1;
@@ -352,7 +435,8 @@ __device__ void Block::init(void* arg) {
     recv_count = 0;
     end_count = 0;
     t_sweep = t_send = t_wait = 0;
-    grids[0] = d.arena + (int64_t)(2 * index) * d.chare_elems + pitch;
+    grids[0] = d.arena + (int64_t)(2 * index) * d.chare_elems +
+               (kHaloRows - 1) * pitch;
     grids[1] = grids[0] + d.chare_elems;
     for (int k = 0; k < N_NEIGHBORS; k++) {
       boundaries[k] = d.staging + (int64_t)(N_NEIGHBORS * index + k) * d.stage_elems;
@@ -386,31 +470,42 @@ __device__ void Block::init(void* arg) {
 }
 
 // Bytes of the halo message toward neighbour k: the dir slot, then `depth`
-// rows/columns of the edge, or one corner cell.
+// rows/columns of the edge, or a (depth-1) x (depth-1) corner block.
 __device__ size_t Block::halo_bytes(int k) const {
-  size_t len = 1;
+  size_t len = (size_t)(depth - 1) * (depth - 1);
   if (k == LEFT || k == RIGHT) len = (size_t)depth * block_height;
   if (k == TOP || k == BOTTOM) len = (size_t)depth * block_width;
   return sizeof(double) + sizeof(double) * len;
 }
 
-// Thread 0: with two-deep halos the four corner cells go to the diagonal
-// neighbours, taken from the top and bottom edges already written (a
-// diagonal neighbour implies both).  bufs[k] = [dir slot | payload].
+// Thread 0: with halos D >= 2 deep the four (D-1) x (D-1) corner blocks go
+// to the diagonal neighbours, taken from the top and bottom edges already
+// written (a diagonal neighbour implies both): the block nearest each
+// corner of the chare, rows in order.  bufs[k] = [dir slot | payload].
+// (A receiver needs cells within D-1 steps of its corner, which the
+// (D-1) x (D-1) block covers; the farthest one is unused.)
 __device__ void Block::fill_corners(double* const* bufs) {
-  if (depth != 2) return;
-  const int bw = block_width;
-  const double* top = bufs[TOP] + 1;
-  const double* bot = bufs[BOTTOM] + 1 + bw;
-  if (neighbor_index[TOP_LEFT] >= 0) bufs[TOP_LEFT][1] = top[0];
-  if (neighbor_index[TOP_RIGHT] >= 0) bufs[TOP_RIGHT][1] = top[bw - 1];
-  if (neighbor_index[BOTTOM_LEFT] >= 0) bufs[BOTTOM_LEFT][1] = bot[0];
-  if (neighbor_index[BOTTOM_RIGHT] >= 0) bufs[BOTTOM_RIGHT][1] = bot[bw - 1];
+  const int D = depth;
+  if (D < 2) return;
+  const int bw = block_width, n = D - 1;
+  const double* top = bufs[TOP] + 1;                 // rows 1 .. D
+  const double* bot = bufs[BOTTOM] + 1 + (int64_t)bw;  // rows bh-D+2 .. bh
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      if (neighbor_index[TOP_LEFT] >= 0)
+        bufs[TOP_LEFT][1 + i * n + j] = top[(int64_t)i * bw + j];
+      if (neighbor_index[TOP_RIGHT] >= 0)
+        bufs[TOP_RIGHT][1 + i * n + j] = top[(int64_t)i * bw + bw - n + j];
+      if (neighbor_index[BOTTOM_LEFT] >= 0)
+        bufs[BOTTOM_LEFT][1 + i * n + j] = bot[(int64_t)i * bw + j];
+      if (neighbor_index[BOTTOM_RIGHT] >= 0)
+        bufs[BOTTOM_RIGHT][1 + i * n + j] = bot[(int64_t)i * bw + bw - n + j];
+    }
 }
 
 __device__ void Block::send_boundaries() {
   // (jacobi2d.cu:152-194) sends tagged with the iteration: four edges
-  // `depth` deep, and with two-deep halos the four corner cells, copied
+  // `depth` deep, and with deeper halos the four corner blocks, copied
   // from the staging buffers
   if (threadIdx.x == 0) fill_corners(boundaries);
   __syncthreads();
@@ -449,18 +544,19 @@ __device__ void Block::recv_ghosts(void* arg) {
       for (int k = threadIdx.x; k < D * bw; k += blockDim.x)
         t[P * (bh + 1 + k / bw) + kColOff + 1 + k % bw] = ghost[k];
       break;
+    // corner blocks of (D-1) x (D-1): rows 2-D .. 0 or bh+1 .. bh+D-1,
+    // columns 2-D .. 0 or bw+1 .. bw+D-1
     case TOP_LEFT:
-      if (threadIdx.x == 0) t[kColOff] = ghost[0];
-      break;
     case TOP_RIGHT:
-      if (threadIdx.x == 0) t[kColOff + bw + 1] = ghost[0];
-      break;
     case BOTTOM_LEFT:
-      if (threadIdx.x == 0) t[P * (bh + 1) + kColOff] = ghost[0];
+    case BOTTOM_RIGHT: {
+      const int n = D - 1;
+      const int i0 = (dir == TOP_LEFT || dir == TOP_RIGHT) ? 2 - D : bh + 1;
+      const int j0 = (dir == TOP_LEFT || dir == BOTTOM_LEFT) ? 2 - D : bw + 1;
+      for (int k = threadIdx.x; k < n * n; k += blockDim.x)
+        t[P * (i0 + k / n) + kColOff + j0 + k % n] = ghost[k];
       break;
-    case BOTTOM_RIGHT:
-      if (threadIdx.x == 0) t[P * (bh + 1) + kColOff + bw + 1] = ghost[0];
-      break;
+    }
     default:
       if (threadIdx.x == 0) charm::rt::raise_error(charm::rt::ERR_BAD_MESSAGE, dir);
       break;
@@ -486,9 +582,9 @@ __device__ void Block::update() {
       t0 = wall_clock64();
       t_wait += t0 - t_mark;
     }
-    // two fused sweeps per exchange with two-deep halos (a lone final
-    // sweep when the count is odd), else one
-    const int steps = (depth == 2 && n_iters - iter >= 2) ? 2 : 1;
+    // `depth` fused sweeps per exchange (fewer in a final round that has
+    // fewer left)
+    const int steps = min(depth, n_iters - iter);
     const bool last = (iter + steps == n_iters);
     double* t = grids[cur];
     double* nt = grids[cur ^ 1];
@@ -527,12 +623,15 @@ __device__ void Block::update() {
     eo.bottom = s_dst[BOTTOM] ? s_dst[BOTTOM] + 1 : nullptr;
     eo.left = s_dst[LEFT] ? s_dst[LEFT] + 1 : nullptr;
     eo.right = s_dst[RIGHT] ? s_dst[RIGHT] + 1 : nullptr;
-    if (steps == 2) {
+    if (steps >= 2) {
       EdgeOut2 eo2 = {eo.top, eo.bottom, eo.left, eo.right};
       const RingSides rs = {neighbor_index[TOP] < 0, neighbor_index[BOTTOM] < 0,
                             neighbor_index[LEFT] < 0,
                             neighbor_index[RIGHT] < 0};
-      sweep2_dispatch(t, nt, P, bw, bh, last, !last, eo2, rs);
+      if (steps == 3)
+        sweep3_dispatch(t, nt, P, bw, bh, last, !last, eo2, rs);
+      else
+        sweep2_dispatch(t, nt, P, bw, bh, last, !last, eo2, rs);
     } else if (ov) {
       sweep_dispatch(t, nt, P, bw, bh, last, eo, 0);
     } else {
@@ -641,7 +740,8 @@ __global__ void hash_chares_kernel(JacobiDesc d, int parity, int r0, int r1,
   const int c = blockIdx.y;
   const ChareGeom g = chare_geom(d.W, d.H, d.cr, d.cc, c);
   const int64_t P = chare_pitch(g.bw);
-  const double* a = d.arena + (int64_t)(2 * c + parity) * d.chare_elems + P;
+  const double* a =
+      d.arena + (int64_t)(2 * c + parity) * d.chare_elems + (kHaloRows - 1) * P;
   const int lr0 = max(r0, g.row0 + 1), lr1 = min(r1, g.row0 + 1 + g.bh);
   const int lc0 = max(c0, g.col0 + 1), lc1 = min(c1, g.col0 + 1 + g.bw);
   uint64_t h = 0;
@@ -777,12 +877,22 @@ int persistent_solve(Handle* h, chm_jacobi_result* res) {
   d.tune_u = h->tune.rows_in_flight;
   d.overlap = h->tune.overlap;
   d.halo_copy = h->tune.halo_copy;
-  // two-deep halos need every chare at least 2 x 2
+  // halos D deep need every chare at least D x D (all of a halo then comes
+  // from the adjacent chares)
   d.depth = h->tune.depth ? h->tune.depth : (ps->min_side >= 2 ? 2 : 1);
-  if (d.depth != 1 && d.depth != 2)
-    return fail(CHM_E_INVALID, "halo depth %d (1 or 2)", d.depth);
-  if (d.depth == 2 && ps->min_side < 2)
-    return fail(CHM_E_INVALID, "halo depth 2 needs chares of at least 2x2");
+  if (d.depth < 1 || d.depth > kHaloRows)
+    return fail(CHM_E_INVALID, "halo depth %d (1, 2 or 3)", d.depth);
+  if (ps->min_side < d.depth)
+    return fail(CHM_E_INVALID, "halo depth %d needs chares of at least %dx%d",
+                d.depth, d.depth, d.depth);
+  if (d.depth == 3 && d.tune_nc) {
+    // shapes of the fused three-sweep pass (sweep3_dispatch)
+    const int nc = d.tune_nc, u = d.tune_u ? d.tune_u : 1;
+    if (!(nc == 1 && (u == 1 || u == 2)))
+      return fail(CHM_E_INVALID,
+                  "halo depth 3 sweeps with panel_chunks x rows_in_flight of "
+                  "1x1 or 1x2 (asked %dx%d)", nc, u);
+  }
   if (d.depth == 2 && d.tune_nc) {
     // shapes of the fused two-sweep pass (sweep2_dispatch)
     const int nc = d.tune_nc, u = d.tune_u ? d.tune_u : 1;
@@ -803,7 +913,7 @@ int persistent_solve(Handle* h, chm_jacobi_result* res) {
     const int rc = chm_run();
     if (rc != CHM_OK) return fail(rc, "persistent run: %s", chm_last_error());
     // one array flip per exchange round
-    const int rounds = d.depth == 2 ? (d.iters + 1) / 2 : d.iters;
+    const int rounds = (d.iters + d.depth - 1) / d.depth;
     h->cur = rounds & 1;
   }
   if (res) {
@@ -848,7 +958,8 @@ int persistent_read_grid(Handle* h, double* out) {
   for (int c = 0; c < ps->n_chares; c++) {
     const ChareGeom g = chare_geom(d.W, d.H, d.cr, d.cc, c);
     const int64_t P = chare_pitch(g.bw);
-    const double* a = d.arena + (int64_t)(2 * c + h->cur) * d.chare_elems + P;
+    const double* a = d.arena + (int64_t)(2 * c + h->cur) * d.chare_elems +
+                      (kHaloRows - 1) * P;
     CHM_TRY_HIP(hipMemcpy2DAsync(
         out + (int64_t)(g.row0 + 1) * GP + g.col0 + 1, GP * sizeof(double),
         a + P + kColOff + 1, P * sizeof(double), (size_t)g.bw * sizeof(double),

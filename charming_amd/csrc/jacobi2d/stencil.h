@@ -515,6 +515,214 @@ __device__ inline void sweep_panel2(const double* __restrict__ in,
   }
 }
 
+// ----------------------------------------------------------------------------
+// Three sweeps fused (temporal blocking, halos three deep)
+// ----------------------------------------------------------------------------
+// The same scheme one stage deeper: u_{k+1} is computed on two extra rows
+// and columns around the output region, u_{k+2} on one, u_{k+3} is stored.
+// Lane 0 holds the columns left of the panel, lane 63 those right of it
+// (M of them in a row with M edge values: c0-1, c0-2, ... / c0+128NC, ...).
+
+template <int NC, int M>
+struct RowE {
+  double2 v[NC];
+  double e[M];
+};
+
+// Stores u' of the pair at columns (c, c+1) of an output row, its residual
+// and its share of outgoing edges `D` deep (chare form; see EdgeOut2 for the
+// layout with D = 2: rows 1..D, rows h-D+1..h, columns 1..D, columns
+// w-D+1..w).
+template <int D, bool RES, bool PACK>
+__device__ inline void store_pair(gdouble* orow, int c, int cend, int w, int h,
+                                  int row, double na, double nb, double a,
+                                  double b, u128acc& acc, const EdgeOut2& eo) {
+  const bool two = c + 1 <= cend, one = c == cend;
+  if (two) {
+    gstore2(orow + c, na, nb);
+    if (RES) {
+      acc_add(acc, res_term(na, a));
+      acc_add(acc, res_term(nb, b));
+    }
+  } else if (one) {
+    orow[c] = na;
+    if (RES) acc_add(acc, res_term(na, a));
+  }
+  if (PACK && (two || one)) {
+    if (row <= D && eo.top) {
+      eo.top[(int64_t)(row - 1) * w + c - 1] = na;
+      if (two) eo.top[(int64_t)(row - 1) * w + c] = nb;
+    }
+    if (row >= h - D + 1 && eo.bottom) {
+      eo.bottom[(int64_t)(row - (h - D + 1)) * w + c - 1] = na;
+      if (two) eo.bottom[(int64_t)(row - (h - D + 1)) * w + c] = nb;
+    }
+    if (eo.left) {
+      if (c <= D) eo.left[(int64_t)(c - 1) * h + row - 1] = na;
+      if (two && c + 1 <= D) eo.left[(int64_t)c * h + row - 1] = nb;
+    }
+    if (eo.right) {
+      if (c >= w - D + 1) eo.right[(int64_t)(c - (w - D + 1)) * h + row - 1] = na;
+      if (two && c + 1 >= w - D + 1)
+        eo.right[(int64_t)(c + 1 - (w - D + 1)) * h + row - 1] = nb;
+    }
+  }
+}
+
+template <int NC>
+struct Panel3 {
+  const gcdouble* in;
+  gdouble* out;
+  int64_t pitch;
+  int w, h, c0, cend, lane;
+  int ecol[3];  // columns this lane loads into e[0..2] (-1000: none)
+  RingSides ring;
+
+  __device__ inline void setup(const double* in_, double* out_, int64_t p,
+                               int w_, int h_, int c0_, int cend_,
+                               RingSides rs) {
+    in = (const gcdouble*)in_;
+    out = (gdouble*)out_;
+    pitch = p;
+    w = w_;
+    h = h_;
+    c0 = c0_;
+    cend = cend_;
+    ring = rs;
+    lane = __lane_id();
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      ecol[j] = -1000;
+      if (lane == 0) ecol[j] = c0 - 1 - j;
+      if (lane == 63 && c0 + 128 * NC + j <= w + 3) ecol[j] = c0 + 128 * NC + j;
+    }
+  }
+  __device__ inline int col(int k) const { return c0 + 128 * k + 2 * lane; }
+
+  __device__ inline void load(RowE<NC, 3>& r, int row) const {
+    const gcdouble* rp = in + (int64_t)row * pitch + kColOff;
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      const int c = col(k);
+      r.v[k] = (c <= w + 3) ? gload2(rp + c) : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) r.e[j] = (ecol[j] > -1000) ? rp[ecol[j]] : 0.0;
+  }
+
+  // One intermediate stage: row q of the next grid from three rows with M
+  // edge values; the result keeps M-1 of them.  Cells on the grid's ring
+  // keep their value.
+  template <int M>
+  __device__ inline void stage(const RowE<NC, M>& up, const RowE<NC, M>& cu,
+                               const RowE<NC, M>& dn, int q,
+                               RowE<NC, M - 1>& o) const {
+    if ((q == 0 && ring.top) || (q == h + 1 && ring.bottom)) {
+#pragma unroll
+      for (int k = 0; k < NC; k++) o.v[k] = cu.v[k];
+#pragma unroll
+      for (int j = 0; j < M - 1; j++) o.e[j] = cu.e[j];
+      return;
+    }
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      const double a = cu.v[k].x, b = cu.v[k].y;
+      double bl = from_left_lane(b);
+      double ar = from_right_lane(a);
+      if (lane == 0) bl = (k == 0) ? cu.e[0] : readlane_d(cu.v[k - 1].y, 63);
+      if (lane == 63)
+        ar = (k == NC - 1) ? cu.e[0] : readlane_d(cu.v[(k + 1) % NC].x, 0);
+      // c + W + E + N + S, then * 0.2 (jacobi2d.cu:332-333)
+      double na = (a + bl + b + up.v[k].x + dn.v[k].x) * 0.2;
+      double nb = (b + a + ar + up.v[k].y + dn.v[k].y) * 0.2;
+      const int c = col(k);
+      if (ring.right) {
+        if (c == w + 1) na = a;
+        if (c + 1 == w + 1) nb = b;
+      }
+      o.v[k] = make_double2(na, nb);
+    }
+    // edge column j: lane 0 holds c0-1-j (west = farther out), lane 63
+    // holds c0+128NC+j (east = farther out)
+#pragma unroll
+    for (int j = 0; j < M - 1; j++) {
+      const double inner =
+          (j == 0) ? ((lane == 0) ? cu.v[0].x : cu.v[NC - 1].y) : cu.e[j - 1];
+      const double outer = cu.e[j + 1];
+      const double wv = (lane == 0) ? outer : inner;
+      const double ev = (lane == 0) ? inner : outer;
+      double v = (cu.e[j] + wv + ev + up.e[j] + dn.e[j]) * 0.2;
+      if (lane == 0 && ring.left && c0 - 1 - j == 0) v = cu.e[j];
+      if (lane == 63 && ring.right && c0 + 128 * NC + j == w + 1) v = cu.e[j];
+      o.e[j] = v;
+    }
+  }
+
+  // The stored stage: row `row` of u_{k+3}.
+  template <bool RES, bool PACK>
+  __device__ inline void outrow(const RowE<NC, 1>& up, const RowE<NC, 1>& cu,
+                                const RowE<NC, 1>& dn, int row, u128acc& acc,
+                                const EdgeOut2& eo) const {
+    gdouble* orow = out + (int64_t)row * pitch + kColOff;
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      const double a = cu.v[k].x, b = cu.v[k].y;
+      double bl = from_left_lane(b);
+      double ar = from_right_lane(a);
+      if (lane == 0) bl = (k == 0) ? cu.e[0] : readlane_d(cu.v[k - 1].y, 63);
+      if (lane == 63)
+        ar = (k == NC - 1) ? cu.e[0] : readlane_d(cu.v[(k + 1) % NC].x, 0);
+      const double na = (a + bl + b + up.v[k].x + dn.v[k].x) * 0.2;
+      const double nb = (b + a + ar + up.v[k].y + dn.v[k].y) * 0.2;
+      store_pair<3, RES, PACK>(orow, col(k), cend, w, h, row, na, nb, a, b,
+                               acc, eo);
+    }
+  }
+};
+
+// Three fused sweeps of rows [r0, r1) of the panel at column c0: reads u_k
+// rows r0-3 .. r1+2, writes u_{k+3} rows r0 .. r1-1, columns c0 ..
+// min(c0+128NC-1, cend).  RES adds the exact residual of the third sweep.
+template <int NC, int U, bool RES, bool PACK>
+__device__ inline void sweep_panel3(const double* __restrict__ in,
+                                    double* __restrict__ out, int64_t pitch,
+                                    int w, int h, int c0, int r0, int r1,
+                                    u128acc& acc, const EdgeOut2& eo,
+                                    RingSides rs, int cend = -1) {
+  if (r0 >= r1) return;
+  constexpr int K = U + 2;
+  Panel3<NC> p;
+  p.setup(in, out, pitch, w, h, c0, cend < 0 ? w : cend, rs);
+  RowE<NC, 3> R[K];
+  RowE<NC, 2> a0, a1, a2;  // rows of u_{k+1}
+  RowE<NC, 1> b0, b1, b2;  // rows of u_{k+2}
+#pragma unroll
+  for (int s = 0; s < K; s++)
+    if (r0 - 3 + s <= r1 + 2) p.load(R[s], r0 - 3 + s);
+  // step t: u_{k+1} row q1 = r0-2+t from u_k rows q1-1 (slot t % K) .. q1+1;
+  // then u_{k+2} row q1-1, then u_{k+3} row q1-2
+  const int n_steps = r1 - r0 + 4;
+  for (int t0 = 0; t0 < n_steps; t0 += K) {
+#pragma unroll
+    for (int s = 0; s < K; s++) {
+      const int t = t0 + s;
+      if (t < n_steps) {
+        const int q1 = r0 - 2 + t;
+        a0 = a1;
+        a1 = a2;
+        p.template stage<3>(R[s], R[(s + 1) % K], R[(s + 2) % K], q1, a2);
+        if (q1 - 1 + K <= r1 + 2) p.load(R[s], q1 - 1 + K);
+        if (t >= 2) {
+          b0 = b1;
+          b1 = b2;
+          p.template stage<2>(a0, a1, a2, q1 - 1, b2);
+          if (t >= 4) p.template outrow<RES, PACK>(b0, b1, b2, q1 - 2, acc, eo);
+        }
+      }
+    }
+  }
+}
+
 // Wave-level reduction of a u128 accumulator to lane 0 (exact).
 __device__ inline u128acc wave_reduce_u128(u128acc a) {
   for (int off = 32; off >= 1; off >>= 1) {

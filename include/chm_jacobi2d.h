@@ -68,7 +68,8 @@ typedef struct chm_jacobi_tuning {
   int overlap;       /* persistent form: 1 = sweep the edges first and send the
                         halos before the interior (default 0; depth 1 only) */
   int depth;         /* persistent form: halo depth = sweeps per exchange,
-                        1 or 2 (0 = 2 when every chare is at least 2x2) */
+                        1, 2 or 3 (every chare at least depth x depth;
+                        0 = 2 when every chare is at least 2x2) */
   int halo_copy;     /* persistent form: 1 = pack halos into staging and send
                         them by copy (invoke); 0 = zero-copy: the sweep writes
                         them into messages allocated in the neighbours'

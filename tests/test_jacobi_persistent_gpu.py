@@ -17,9 +17,12 @@ def cam():
 
 
 def expected_messages(cr, cc, iters, depth):
+    # init + end message per chare, and per exchange round (one per `depth`
+    # sweeps, the first sent by init) both directions of every edge and, with
+    # halos at least two deep, of every corner
     edges = 2 * (cr * (cc - 1) + cc * (cr - 1))
-    corners = 4 * (cr - 1) * (cc - 1) if depth == 2 else 0
-    rounds = (iters + 1) // 2 if depth == 2 else iters
+    corners = 4 * (cr - 1) * (cc - 1) if depth >= 2 else 0
+    rounds = -(-iters // depth)
     return 2 * cr * cc + rounds * (edges + corners)
 
 
@@ -90,12 +93,12 @@ def test_default_decomposition_uses_all_pes(cam):
 
 
 @pytest.mark.parametrize("halo_copy", [0, 1])
-@pytest.mark.parametrize("depth", [1, 2])
+@pytest.mark.parametrize("depth", [1, 2, 3])
 @pytest.mark.parametrize("W,H,cr,cc,iters", [
     (300, 200, 3, 5, 17), (257, 129, 4, 3, 10), (64, 64, 2, 2, 1),
     (100, 100, 5, 5, 2), (130, 70, 1, 1, 9), (500, 40, 2, 7, 5)])
 def test_halo_depths_bit_exact(cam, depth, halo_copy, W, H, cr, cc, iters):
-    # depth 1/2 = sweeps per exchange; halo_copy 0 = zero-copy halo messages
+    # depth 1/2/3 = sweeps per exchange; halo_copy 0 = zero-copy halo messages
     # (allocated in the neighbours' inboxes, written by the sweep), 1 = staged
     ref, res = O.jacobi_run(W, H, iters, seed=W + H)
     with cam.Jacobi2D(W, H, iters, form="persistent", seed=W + H,
@@ -118,9 +121,42 @@ def test_fused_sweep_shapes(cam, nc, u):
         assert np.array_equal(j.grid(), ref) and r.residual == res
 
 
+@pytest.mark.parametrize("W,H,cr,cc,iters", [
+    (64, 9, 3, 2, 7),      # three-row chares (the thinnest depth 3 allows)
+    (40, 40, 8, 8, 7),     # 64 chares of 5x5: rounds of 3, 3, 1
+    (129, 257, 2, 2, 4),   # rounds of 3, 1
+    (512, 512, 4, 4, 100)])
+def test_depth3_edge_cases(cam, W, H, cr, cc, iters):
+    ref, res = O.jacobi_run(W, H, iters, seed=W * 3 + H)
+    with cam.Jacobi2D(W, H, iters, form="persistent", seed=W * 3 + H,
+                      chares=(cr, cc)) as j:
+        j.set_tuning(depth=3)
+        r = j.solve()
+        assert np.array_equal(j.grid(), ref)
+        assert r.residual == res and r.hash == O.grid_hash(ref)
+        assert r.messages == expected_messages(cr, cc, iters, 3)
+
+
+@pytest.mark.parametrize("nc,u,iters", [(1, 1, 12), (1, 2, 13), (1, 1, 14)])
+def test_fused_sweep3_shapes(cam, nc, u, iters):
+    W, H = 1000, 600
+    ref, res = O.jacobi_run(W, H, iters, seed=23, mode=1)
+    with cam.Jacobi2D(W, H, iters, form="persistent", seed=23, chares=(3, 2),
+                      init_mode=cam.INIT_REFERENCE) as j:
+        j.set_tuning(nc, u, depth=3)
+        r = j.solve()
+        assert np.array_equal(j.grid(), ref) and r.residual == res
+
+
 def test_unsupported_fused_shape_is_refused(cam):
     with cam.Jacobi2D(256, 256, 4, form="persistent", seed=1, chares=(2, 2)) as j:
         j.set_tuning(4, 1, depth=2)
+        with pytest.raises(cam.ChmError):
+            j.solve()
+        j.set_tuning(2, 1, depth=3)
+        with pytest.raises(cam.ChmError):
+            j.solve()
+        j.set_tuning(depth=4)
         with pytest.raises(cam.ChmError):
             j.solve()
 
@@ -133,5 +169,10 @@ def test_thin_chares_fall_back_to_depth_one(cam):
         assert np.array_equal(j.grid(), ref) and r.residual == res
         j.reset()
         j.set_tuning(depth=2)
+        with pytest.raises(cam.ChmError):
+            j.solve()
+    # two-row chares: depth 3 is refused
+    with cam.Jacobi2D(64, 8, 6, form="persistent", seed=5, chares=(4, 2)) as j:
+        j.set_tuning(depth=3)
         with pytest.raises(cam.ChmError):
             j.solve()
