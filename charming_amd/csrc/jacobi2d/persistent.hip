@@ -18,7 +18,7 @@
 //     is harmless only for square chare grids);
 //   - the last row/column of chares takes the remainder of an uneven split;
 //   - a chare with no neighbours iterates without waiting;
-//   - temporal blocking (halo depth D = 2, the default, or 3): chares
+//   - temporal blocking (halo depth D = 2 .. 8): chares
 //     exchange halos D cells deep (four edges of D rows/columns plus four
 //     (D-1) x (D-1) corner blocks from the diagonal neighbours) once per D
 //     sweeps, and sweep D times in one pass over their block with the
@@ -58,16 +58,19 @@ struct JacobiDesc {
   uint64_t* phase;        // per chare: wall-clock ticks in sweep, send, wait
   int W, H, cr, cc, iters;
   int tune_nc, tune_u;
+  int tune_band;          // deep fused sweeps: rows per band, < 0 = one
+                          // contiguous share of the block per wave, 0 = by
+                          // chare height
   int overlap;            // sweep edges first, send, then the interior
   uint64_t* recv_ticks;   // per chare: ticks inside recv_ghosts (unpack)
-  int depth;              // halo depth = sweeps per exchange (1, 2 or 3)
+  int depth;              // halo depth = sweeps per exchange (1 .. 8)
   int halo_copy;          // 1: halos staged and copied by invoke (else the
                           // sweep writes them into the messages in place)
 };
 
-// Chare arrays always hold halos three deep: rows -2 .. bh+3 of
-// pitch_for_depth(bw, 3) doubles; grids[] point at row 0.
-constexpr int kHaloRows = 3;
+// Chare arrays always hold halos of the deepest supported depth, eight: rows
+// -7 .. bh+8 of pitch_for_depth(bw, 8) doubles; grids[] point at row 0.
+constexpr int kHaloRows = 8;
 __host__ __device__ inline int64_t chare_pitch(int bw) {
   return pitch_for_depth(bw, kHaloRows);
 }
@@ -299,37 +302,60 @@ __device__ inline T* wuni(T* p) {
   return (T*)wuni((int64_t)p);
 }
 
-// Three fused sweeps of the whole block (halos three deep), panels x bands.
-template <int NC, int U, bool RES, bool PACK>
-__device__ __noinline__ void chare_sweep3(const double* in, double* out,
+// D >= 3 fused sweeps of the whole block (halos D deep), overlapping
+// windows x bands.  The outgoing edges are packed afterwards (pack_edges):
+// kept out of the sweep, they would cost its inner loop registers it needs
+// for the intermediate grids.
+template <int D, bool RES>
+__device__ __noinline__ void chare_sweepD(const double* in, double* out,
                                           int64_t P, int bw, int bh,
-                                          EdgeOut2 eo, RingSides rs) {
+                                          RingSides rs) {
   in = wuni(in);
   out = wuni(out);
   P = wuni(P);
   bw = wuni(bw);
   bh = wuni(bh);
-  eo.top = wuni(eo.top);
-  eo.bottom = wuni(eo.bottom);
-  eo.left = wuni(eo.left);
-  eo.right = wuni(eo.right);
   rs.top = wuni((int)rs.top);
   rs.bottom = wuni((int)rs.bottom);
   rs.left = wuni((int)rs.left);
   rs.right = wuni((int)rs.right);
+  // (the wave index stays a vector value: moved to a scalar register, the
+  // row bookkeeping derived from it raises scalar-register pressure enough
+  // to spill in the deeper sweeps, measured slower)
   const int wave = threadIdx.x / 64, WV = blockDim.x / 64;
-  const int pw = 128 * NC;
-  const int S = (bw + pw - 1) / pw;
-  int nb = WV / S;
-  if (nb < 1) nb = 1;
-  if (nb > bh) nb = bh;
-  const int br = (bh + nb - 1) / nb;
+  constexpr int OW = window_out_cols(D);
+  const int S = (bw + OW - 1) / OW;
   u128acc acc = {0, 0};
-  for (int t = wave; t < S * nb; t += WV) {
-    const int panel = t % S, b = t / S;
-    const int r0 = 1 + b * br, r1 = min(r0 + br, bh + 1);
-    sweep_panel3<NC, U, RES, PACK>(in, out, P, bw, bh, 1 + panel * pw, r0, r1,
-                                   acc, eo, rs);
+  const EdgeOut2 none = {nullptr, nullptr, nullptr, nullptr};
+  // bands of 256 rows for tall chares (measured best on 1024-row chares);
+  // shorter ones are cut into contiguous shares
+  int band = wuni(g_jdesc.tune_band);
+  if (band == 0) band = bh >= 1024 ? 256 : -1;
+  if (band > 0) {
+    // bands of `band` rows, windows of a band side by side: waves take the
+    // tiles round-robin, so neighbouring windows (which read overlapping
+    // columns) run at the same rows at about the same time
+    const int nb = (bh + band - 1) / band;
+    for (int t = wave; t < S * nb; t += WV) {
+      const int panel = t % S, b = t / S;
+      const int r0 = 1 + b * band, r1 = min(r0 + band, bh + 1);
+      sweep_panelD<D, RES, false>(in, out, P, bw, bh, 1 + panel * OW, r0, r1,
+                                  acc, none, rs);
+    }
+  } else {
+    // the S windows x bh rows, window after window, cut into one contiguous
+    // range per wave: equal work whatever S is, at most a few bands each
+    const int64_t total = (int64_t)S * bh;
+    const int64_t per = (total + WV - 1) / WV;
+    int64_t pos = (int64_t)wave * per;
+    const int64_t end = min(total, pos + per);
+    while (pos < end) {
+      const int panel = (int)(pos / bh), r = (int)(pos % bh);
+      const int len = (int)min((int64_t)(bh - r), end - pos);
+      sweep_panelD<D, RES, false>(in, out, P, bw, bh, 1 + panel * OW, 1 + r,
+                                  1 + r + len, acc, none, rs);
+      pos += len;
+    }
   }
   if (RES) {
     acc = wave_reduce_u128(acc);
@@ -337,34 +363,51 @@ __device__ __noinline__ void chare_sweep3(const double* in, double* out,
   }
 }
 
-__device__ void sweep3_dispatch(const double* in, double* out, int64_t P,
-                                int bw, int bh, bool res, bool pack,
-                                const EdgeOut2& eo, RingSides rs) {
-  // 1 chunk x 1 row in flight (three rings of rows live in registers)
-  int nc = g_jdesc.tune_nc, u = g_jdesc.tune_u;
-  if (nc == 0) {
-    nc = 1;
-    u = 1;
+// `steps` (3 .. kHaloRows) fused sweeps in one pass.
+__device__ void sweepD_dispatch(int steps, const double* in, double* out,
+                                int64_t P, int bw, int bh, bool res,
+                                RingSides rs) {
+#define CHM_CASED(D_)                                            \
+  case D_:                                                       \
+    if (res)                                                     \
+      chare_sweepD<D_, true>(in, out, P, bw, bh, rs);            \
+    else                                                         \
+      chare_sweepD<D_, false>(in, out, P, bw, bh, rs);           \
+    break;
+  switch (steps) {
+    CHM_CASED(3) CHM_CASED(4) CHM_CASED(5) CHM_CASED(6) CHM_CASED(7)
+    CHM_CASED(8)
+    default: break;
   }
-  if (u == 0) u = 1;
-#define CHM_CASE3(NC_, U_)                                                   \
-  if (nc == NC_ && u == U_) {                                                \
-    if (res)                                                                 \
-      chare_sweep3<NC_, U_, true, false>(in, out, P, bw, bh, eo, rs);        \
-    else if (pack)                                                           \
-      chare_sweep3<NC_, U_, false, true>(in, out, P, bw, bh, eo, rs);        \
-    else                                                                     \
-      chare_sweep3<NC_, U_, false, false>(in, out, P, bw, bh, eo, rs);       \
-    return;                                                                  \
+#undef CHM_CASED
+}
+
+// The whole workgroup: the outgoing edges of the array g, D deep, into the
+// messages' payloads (layout of EdgeOut2: top = rows 1..D, bottom = rows
+// bh-D+1..bh, w each; left = columns 1..D, right = columns bw-D+1..bw, h
+// each).  Other waves of the workgroup wrote g just before, partly with
+// streaming stores, so the loads go to L2 (agent-scope atomic loads) rather
+// than trust the CU's L1.
+__device__ __noinline__ void pack_edges(const double* g, int64_t P, int bw,
+                                        int bh, int D, EdgeOut2 eo) {
+  auto ld = [](const double* p) {
+    const uint64_t v = __hip_atomic_load((const uint64_t*)p, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+    return __longlong_as_double((long long)v);
+  };
+  const int n_row = D * bw, n_col = D * bh;
+  for (int k = threadIdx.x; k < n_row; k += blockDim.x) {
+    const int i = k / bw, j = 1 + k % bw;
+    if (eo.top) eo.top[k] = ld(g + (int64_t)(1 + i) * P + kColOff + j);
+    if (eo.bottom)
+      eo.bottom[k] = ld(g + (int64_t)(bh - D + 1 + i) * P + kColOff + j);
   }
-  CHM_CASE3(1, 1) CHM_CASE3(1, 2)
-#undef CHM_CASE3
-  if (res)
-    chare_sweep3<1, 1, true, false>(in, out, P, bw, bh, eo, rs);
-  else if (pack)
-    chare_sweep3<1, 1, false, true>(in, out, P, bw, bh, eo, rs);
-  else
-    chare_sweep3<1, 1, false, false>(in, out, P, bw, bh, eo, rs);
+  for (int k = threadIdx.x; k < n_col; k += blockDim.x) {
+    const int j = k / bh, i = 1 + k % bh;
+    if (eo.left) eo.left[k] = ld(g + (int64_t)i * P + kColOff + 1 + j);
+    if (eo.right)
+      eo.right[k] = ld(g + (int64_t)i * P + kColOff + bw - D + 1 + j);
+  }
 }
 
 __device__ void sweep2_dispatch(const double* in, double* out, int64_t P,
@@ -628,9 +671,13 @@ __device__ void Block::update() {
       const RingSides rs = {neighbor_index[TOP] < 0, neighbor_index[BOTTOM] < 0,
                             neighbor_index[LEFT] < 0,
                             neighbor_index[RIGHT] < 0};
-      if (steps == 3)
-        sweep3_dispatch(t, nt, P, bw, bh, last, !last, eo2, rs);
-      else
+      if (steps >= 3) {
+        sweepD_dispatch(steps, t, nt, P, bw, bh, last, rs);
+        if (!last) {
+          __syncthreads();
+          pack_edges(nt, P, bw, bh, steps, eo2);
+        }
+      } else
         sweep2_dispatch(t, nt, P, bw, bh, last, !last, eo2, rs);
     } else if (ov) {
       sweep_dispatch(t, nt, P, bw, bh, last, eo, 0);
@@ -875,24 +922,32 @@ int persistent_solve(Handle* h, chm_jacobi_result* res) {
   // zero = choose per chare width on the device
   d.tune_nc = h->tune.panel_chunks;
   d.tune_u = h->tune.rows_in_flight;
+  d.tune_band = h->tune.band_rows;
   d.overlap = h->tune.overlap;
   d.halo_copy = h->tune.halo_copy;
   // halos D deep need every chare at least D x D (all of a halo then comes
   // from the adjacent chares)
-  d.depth = h->tune.depth ? h->tune.depth : (ps->min_side >= 2 ? 2 : 1);
+  // (default: as deep as the chares allow, up to kHaloRows: fewer exchange
+  // rounds and fewer HBM passes, faster on every decomposition measured,
+  // from 128^2 to 1024^2 chares.  The deep pass has one sweep shape, so a
+  // request for another shape keeps the two-sweep pass.)
+  if (h->tune.depth) {
+    d.depth = h->tune.depth;
+  } else if (d.tune_nc <= 1 && d.tune_u <= 1) {
+    d.depth = std::min(ps->min_side, kHaloRows);
+  } else {
+    d.depth = ps->min_side >= 2 ? 2 : 1;
+  }
   if (d.depth < 1 || d.depth > kHaloRows)
-    return fail(CHM_E_INVALID, "halo depth %d (1, 2 or 3)", d.depth);
+    return fail(CHM_E_INVALID, "halo depth %d (1 .. %d)", d.depth, kHaloRows);
   if (ps->min_side < d.depth)
     return fail(CHM_E_INVALID, "halo depth %d needs chares of at least %dx%d",
                 d.depth, d.depth, d.depth);
-  if (d.depth == 3 && d.tune_nc) {
-    // shapes of the fused three-sweep pass (sweep3_dispatch)
-    const int nc = d.tune_nc, u = d.tune_u ? d.tune_u : 1;
-    if (!(nc == 1 && (u == 1 || u == 2)))
-      return fail(CHM_E_INVALID,
-                  "halo depth 3 sweeps with panel_chunks x rows_in_flight of "
-                  "1x1 or 1x2 (asked %dx%d)", nc, u);
-  }
+  if (d.depth >= 3 && (d.tune_nc > 1 || d.tune_u > 1))
+    // the deep fused pass (sweepD_dispatch) has one shape
+    return fail(CHM_E_INVALID,
+                "halo depth %d sweeps with panel_chunks x rows_in_flight of "
+                "1x1 only (asked %dx%d)", d.depth, d.tune_nc, d.tune_u);
   if (d.depth == 2 && d.tune_nc) {
     // shapes of the fused two-sweep pass (sweep2_dispatch)
     const int nc = d.tune_nc, u = d.tune_u ? d.tune_u : 1;

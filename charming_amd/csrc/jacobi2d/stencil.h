@@ -516,18 +516,11 @@ __device__ inline void sweep_panel2(const double* __restrict__ in,
 }
 
 // ----------------------------------------------------------------------------
-// Three sweeps fused (temporal blocking, halos three deep)
+// D sweeps fused (temporal blocking, halos D deep)
 // ----------------------------------------------------------------------------
-// The same scheme one stage deeper: u_{k+1} is computed on two extra rows
-// and columns around the output region, u_{k+2} on one, u_{k+3} is stored.
-// Lane 0 holds the columns left of the panel, lane 63 those right of it
-// (M of them in a row with M edge values: c0-1, c0-2, ... / c0+128NC, ...).
-
-template <int NC, int M>
-struct RowE {
-  double2 v[NC];
-  double e[M];
-};
+// The same scheme D-1 stages deep: the grid after s sweeps is computed on
+// D-s extra rows and columns around the output region, the grid after D
+// sweeps is stored.
 
 // Stores u' of the pair at columns (c, c+1) of an output row, its residual
 // and its share of outgoing edges `D` deep (chare form; see EdgeOut2 for the
@@ -569,155 +562,192 @@ __device__ inline void store_pair(gdouble* orow, int c, int cend, int w, int h,
   }
 }
 
-template <int NC>
-struct Panel3 {
+// Deep fusion works on overlapping windows: a wave holds 128 columns of a
+// row (lane l: the pair at w0+2l, w0+2l+1), each sweep leaves one more
+// column at either end without a valid neighbour, and after D sweeps the
+// middle 128-2H columns (H = D rounded up to even, so windows stay 16-byte
+// aligned) are stored.  Neighbouring windows overlap by 2H columns; the
+// lane shifts need no boundary fix-ups.
+struct RowW {
+  double2 v;
+};
+
+// Lane l gets lane l-1's v (l+1's for lane_shl1); the end lane gets 0.
+__device__ inline double lane_shr1z(double v) {
+  const long long x = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)x, 0x138, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(x >> 32), 0x138, 0xf, 0xf, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ inline double lane_shl1z(double v) {
+  const long long x = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)x, 0x130, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(x >> 32), 0x130, 0xf, 0xf, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Output columns per window for D fused sweeps.
+__host__ __device__ constexpr int window_out_cols(int D) {
+  return 128 - 2 * (D + (D & 1));
+}
+
+struct PanelW {
   const gcdouble* in;
   gdouble* out;
   int64_t pitch;
-  int w, h, c0, cend, lane;
-  int ecol[3];  // columns this lane loads into e[0..2] (-1000: none)
+  int w, h, c, cend;
+  bool vload;  // this lane's pair lies within the columns read
+  bool vstore; // this lane's pair lies within the window's output columns
+  // cells of this lane on the grid's ring (they keep their value), and
+  // whether the wave has any (a uniform branch skips the selects if not)
+  bool ring_a, ring_b, ringcols;
   RingSides ring;
 
+  // the window whose output starts at column c0 (odd)
   __device__ inline void setup(const double* in_, double* out_, int64_t p,
-                               int w_, int h_, int c0_, int cend_,
-                               RingSides rs) {
+                               int w_, int h_, int c0, int cend_,
+                               RingSides rs, int D) {
     in = (const gcdouble*)in_;
     out = (gdouble*)out_;
     pitch = p;
     w = w_;
     h = h_;
-    c0 = c0_;
     cend = cend_;
     ring = rs;
-    lane = __lane_id();
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-      ecol[j] = -1000;
-      if (lane == 0) ecol[j] = c0 - 1 - j;
-      if (lane == 63 && c0 + 128 * NC + j <= w + 3) ecol[j] = c0 + 128 * NC + j;
-    }
-  }
-  __device__ inline int col(int k) const { return c0 + 128 * k + 2 * lane; }
-
-  __device__ inline void load(RowE<NC, 3>& r, int row) const {
-    const gcdouble* rp = in + (int64_t)row * pitch + kColOff;
-#pragma unroll
-    for (int k = 0; k < NC; k++) {
-      const int c = col(k);
-      r.v[k] = (c <= w + 3) ? gload2(rp + c) : make_double2(0.0, 0.0);
-    }
-#pragma unroll
-    for (int j = 0; j < 3; j++) r.e[j] = (ecol[j] > -1000) ? rp[ecol[j]] : 0.0;
+    const int H = D + (D & 1);
+    const int lane = __lane_id();
+    c = c0 - H + 2 * lane;
+    vload = c <= w + D;
+    vstore = lane >= H / 2 && lane < 64 - H / 2;
+    ring_a = (rs.left && c == 0) || (rs.right && c == w + 1);
+    ring_b = (rs.left && c + 1 == 0) || (rs.right && c + 1 == w + 1);
+    ringcols = __ballot(ring_a || ring_b) != 0;
   }
 
-  // One intermediate stage: row q of the next grid from three rows with M
-  // edge values; the result keeps M-1 of them.  Cells on the grid's ring
-  // keep their value.
-  template <int M>
-  __device__ inline void stage(const RowE<NC, M>& up, const RowE<NC, M>& cu,
-                               const RowE<NC, M>& dn, int q,
-                               RowE<NC, M - 1>& o) const {
-    if ((q == 0 && ring.top) || (q == h + 1 && ring.bottom)) {
-#pragma unroll
-      for (int k = 0; k < NC; k++) o.v[k] = cu.v[k];
-#pragma unroll
-      for (int j = 0; j < M - 1; j++) o.e[j] = cu.e[j];
+  // row `row` of the input; lanes past column w+D keep what they held
+  __device__ inline void load(RowW& r, int row) const {
+    if (vload) r.v = gload2(in + (int64_t)row * pitch + kColOff + c);
+  }
+
+  // One sweep: row q of the next grid from three rows of the current one.
+  // CHECK = false is for rows and waves that meet no ring cell.
+  template <bool CHECK>
+  __device__ inline void stage(const RowW& up, const RowW& cu, const RowW& dn,
+                               int q, RowW& o) const {
+    if (CHECK && ((q == 0 && ring.top) || (q == h + 1 && ring.bottom))) {
+      o = cu;
       return;
     }
-#pragma unroll
-    for (int k = 0; k < NC; k++) {
-      const double a = cu.v[k].x, b = cu.v[k].y;
-      double bl = from_left_lane(b);
-      double ar = from_right_lane(a);
-      if (lane == 0) bl = (k == 0) ? cu.e[0] : readlane_d(cu.v[k - 1].y, 63);
-      if (lane == 63)
-        ar = (k == NC - 1) ? cu.e[0] : readlane_d(cu.v[(k + 1) % NC].x, 0);
-      // c + W + E + N + S, then * 0.2 (jacobi2d.cu:332-333)
-      double na = (a + bl + b + up.v[k].x + dn.v[k].x) * 0.2;
-      double nb = (b + a + ar + up.v[k].y + dn.v[k].y) * 0.2;
-      const int c = col(k);
-      if (ring.right) {
-        if (c == w + 1) na = a;
-        if (c + 1 == w + 1) nb = b;
-      }
-      o.v[k] = make_double2(na, nb);
+    const double a = cu.v.x, b = cu.v.y;
+    const double bl = lane_shr1z(b);  // west of a
+    const double ar = lane_shl1z(a);  // east of b
+    // c + W + E + N + S, then * 0.2 (jacobi2d.cu:332-333)
+    double na = (a + bl + b + up.v.x + dn.v.x) * 0.2;
+    double nb = (b + a + ar + up.v.y + dn.v.y) * 0.2;
+    if (CHECK && ringcols) {
+      if (ring_a) na = a;
+      if (ring_b) nb = b;
     }
-    // edge column j: lane 0 holds c0-1-j (west = farther out), lane 63
-    // holds c0+128NC+j (east = farther out)
-#pragma unroll
-    for (int j = 0; j < M - 1; j++) {
-      const double inner =
-          (j == 0) ? ((lane == 0) ? cu.v[0].x : cu.v[NC - 1].y) : cu.e[j - 1];
-      const double outer = cu.e[j + 1];
-      const double wv = (lane == 0) ? outer : inner;
-      const double ev = (lane == 0) ? inner : outer;
-      double v = (cu.e[j] + wv + ev + up.e[j] + dn.e[j]) * 0.2;
-      if (lane == 0 && ring.left && c0 - 1 - j == 0) v = cu.e[j];
-      if (lane == 63 && ring.right && c0 + 128 * NC + j == w + 1) v = cu.e[j];
-      o.e[j] = v;
-    }
+    o.v = make_double2(na, nb);
   }
 
-  // The stored stage: row `row` of u_{k+3}.
-  template <bool RES, bool PACK>
-  __device__ inline void outrow(const RowE<NC, 1>& up, const RowE<NC, 1>& cu,
-                                const RowE<NC, 1>& dn, int row, u128acc& acc,
+  // The stored sweep: row `row` of the output, its residual against the
+  // input of the sweep, and the outgoing edges D deep.
+  template <int D, bool RES, bool PACK>
+  __device__ inline void outrow(const RowW& up, const RowW& cu, const RowW& dn,
+                                int row, u128acc& acc,
                                 const EdgeOut2& eo) const {
-    gdouble* orow = out + (int64_t)row * pitch + kColOff;
-#pragma unroll
-    for (int k = 0; k < NC; k++) {
-      const double a = cu.v[k].x, b = cu.v[k].y;
-      double bl = from_left_lane(b);
-      double ar = from_right_lane(a);
-      if (lane == 0) bl = (k == 0) ? cu.e[0] : readlane_d(cu.v[k - 1].y, 63);
-      if (lane == 63)
-        ar = (k == NC - 1) ? cu.e[0] : readlane_d(cu.v[(k + 1) % NC].x, 0);
-      const double na = (a + bl + b + up.v[k].x + dn.v[k].x) * 0.2;
-      const double nb = (b + a + ar + up.v[k].y + dn.v[k].y) * 0.2;
-      store_pair<3, RES, PACK>(orow, col(k), cend, w, h, row, na, nb, a, b,
-                               acc, eo);
-    }
+    const double a = cu.v.x, b = cu.v.y;
+    const double bl = lane_shr1z(b);
+    const double ar = lane_shl1z(a);
+    const double na = (a + bl + b + up.v.x + dn.v.x) * 0.2;
+    const double nb = (b + a + ar + up.v.y + dn.v.y) * 0.2;
+    if (vstore)
+      store_pair<D, RES, PACK>(out + (int64_t)row * pitch + kColOff, c, cend,
+                               w, h, row, na, nb, a, b, acc, eo);
   }
 };
 
-// Three fused sweeps of rows [r0, r1) of the panel at column c0: reads u_k
-// rows r0-3 .. r1+2, writes u_{k+3} rows r0 .. r1-1, columns c0 ..
-// min(c0+128NC-1, cend).  RES adds the exact residual of the third sweep.
-template <int NC, int U, bool RES, bool PACK>
-__device__ inline void sweep_panel3(const double* __restrict__ in,
+// D fused sweeps (D >= 3) of rows [r0, r1) of the window whose output
+// starts at column c0 (odd): reads the input rows r0-D .. r1+D-1 (columns
+// c0-H .. c0-H+127, those up to w+D), writes the output rows r0 .. r1-1,
+// columns c0 .. min(c0+window_out_cols(D)-1, cend).  The input rows sit in
+// a ring of three register slots (one row in flight), and each intermediate
+// grid keeps its three live rows in a ring of its own; ring slots are
+// static indices because the step loop is unrolled by three.  RES adds the
+// exact residual of the last sweep.
+template <int D, bool RES, bool PACK>
+__device__ inline void sweep_panelD(const double* __restrict__ in,
                                     double* __restrict__ out, int64_t pitch,
                                     int w, int h, int c0, int r0, int r1,
                                     u128acc& acc, const EdgeOut2& eo,
                                     RingSides rs, int cend = -1) {
+  static_assert(D >= 3 && D <= 14, "deep fused sweep: 3 <= D <= 14");
   if (r0 >= r1) return;
-  constexpr int K = U + 2;
-  Panel3<NC> p;
-  p.setup(in, out, pitch, w, h, c0, cend < 0 ? w : cend, rs);
-  RowE<NC, 3> R[K];
-  RowE<NC, 2> a0, a1, a2;  // rows of u_{k+1}
-  RowE<NC, 1> b0, b1, b2;  // rows of u_{k+2}
+  PanelW p;
+  p.setup(in, out, pitch, w, h, c0, cend < 0 ? w : cend, rs, D);
+  RowW R[3];          // input rows
+  RowW S[D - 1][3];   // S[s-1]: rows of the grid after s sweeps
 #pragma unroll
-  for (int s = 0; s < K; s++)
-    if (r0 - 3 + s <= r1 + 2) p.load(R[s], r0 - 3 + s);
-  // step t: u_{k+1} row q1 = r0-2+t from u_k rows q1-1 (slot t % K) .. q1+1;
-  // then u_{k+2} row q1-1, then u_{k+3} row q1-2
-  const int n_steps = r1 - r0 + 4;
-  for (int t0 = 0; t0 < n_steps; t0 += K) {
+  for (int k = 0; k < 3; k++) {
+    R[k].v = make_double2(0.0, 0.0);
+    if (r0 - D + k <= r1 + D - 1) p.load(R[k], r0 - D + k);
+  }
+  // step t: the first sweep gives row q1 = r0-D+1+t from input rows q1-1
+  // (slot t % 3), q1, q1+1; sweep s >= 2 gives row q1-(s-1) once t >= 2(s-1)
+  // from rows made at steps t-2, t-1, t; the last one is stored
+  const int n_steps = r1 - r0 + 2 * D - 2;
+  // Steps [tA, tB) form the steady state: every sweep is under way, no row
+  // of the grid's ring is met (the ring rows 0 and h+1 only occur among the
+  // first 2D-2 and the last D-1 steps) and no load runs past the band, so
+  // those steps run without guards or ring checks unless the wave holds a
+  // ring column.  Both bounds are multiples of three, as the ring slots
+  // require.
+  const int tA = (2 * D - 2 + 2) / 3 * 3;
+  int tB = tA + (n_steps - (D - 1) - tA) / 3 * 3;
+  if (p.ringcols || tB < tA) tB = tA;
+  auto guarded = [&](int t0) {
 #pragma unroll
-    for (int s = 0; s < K; s++) {
-      const int t = t0 + s;
+    for (int k = 0; k < 3; k++) {
+      const int t = t0 + k;
       if (t < n_steps) {
-        const int q1 = r0 - 2 + t;
-        a0 = a1;
-        a1 = a2;
-        p.template stage<3>(R[s], R[(s + 1) % K], R[(s + 2) % K], q1, a2);
-        if (q1 - 1 + K <= r1 + 2) p.load(R[s], q1 - 1 + K);
-        if (t >= 2) {
-          b0 = b1;
-          b1 = b2;
-          p.template stage<2>(a0, a1, a2, q1 - 1, b2);
-          if (t >= 4) p.template outrow<RES, PACK>(b0, b1, b2, q1 - 2, acc, eo);
-        }
+        const int q1 = r0 - D + 1 + t;
+        p.template stage<true>(R[k], R[(k + 1) % 3], R[(k + 2) % 3], q1,
+                               S[0][k]);
+        if (q1 + 2 <= r1 + D - 1) p.load(R[k], q1 + 2);
+#pragma unroll
+        for (int s = 2; s < D; s++)
+          if (t >= 2 * (s - 1))
+            p.template stage<true>(S[s - 2][(k + 1) % 3],
+                                   S[s - 2][(k + 2) % 3], S[s - 2][k],
+                                   q1 - (s - 1), S[s - 1][k]);
+        if (t >= 2 * (D - 1))
+          p.template outrow<D, RES, PACK>(S[D - 2][(k + 1) % 3],
+                                          S[D - 2][(k + 2) % 3], S[D - 2][k],
+                                          q1 - (D - 1), acc, eo);
+      }
+    }
+  };
+  // the first tA steps, then the steady state, then the rest (one loop for
+  // the guarded steps, so their code exists once)
+  int t0 = 0;
+  for (;;) {
+    for (; t0 < n_steps && !(t0 >= tA && t0 < tB); t0 += 3) guarded(t0);
+    if (t0 >= n_steps) break;
+    for (; t0 < tB; t0 += 3) {
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const int q1 = r0 - D + 1 + t0 + k;
+        p.template stage<false>(R[k], R[(k + 1) % 3], R[(k + 2) % 3], q1,
+                                S[0][k]);
+        p.load(R[k], q1 + 2);
+#pragma unroll
+        for (int s = 2; s < D; s++)
+          p.template stage<false>(S[s - 2][(k + 1) % 3], S[s - 2][(k + 2) % 3],
+                                  S[s - 2][k], q1 - (s - 1), S[s - 1][k]);
+        p.template outrow<D, RES, PACK>(S[D - 2][(k + 1) % 3],
+                                        S[D - 2][(k + 2) % 3], S[D - 2][k],
+                                        q1 - (D - 1), acc, eo);
       }
     }
   }

@@ -63,13 +63,17 @@ typedef struct chm_jacobi_result {
 typedef struct chm_jacobi_tuning {
   int panel_chunks;  /* 1, 2, 4 or 8 */
   int rows_in_flight;/* 1, 2, 4 or 8 */
-  int band_rows;     /* rows per wave band (plain form) */
+  int band_rows;     /* rows per wave band (plain form; persistent form with
+                        halos 3+ deep: < 0 = one contiguous share of the
+                        chare per wave, 0 = 256 for chares of 1024+ rows,
+                        else contiguous shares) */
   int wg_waves;      /* waves per workgroup (plain form) */
   int overlap;       /* persistent form: 1 = sweep the edges first and send the
                         halos before the interior (default 0; depth 1 only) */
   int depth;         /* persistent form: halo depth = sweeps per exchange,
-                        1, 2 or 3 (every chare at least depth x depth;
-                        0 = 2 when every chare is at least 2x2) */
+                        1 .. 8 (every chare at least depth x depth; 0 = as
+                        deep as the smallest chare allows, up to 8, with
+                        the default or 1x1 sweep shape, else 2) */
   int halo_copy;     /* persistent form: 1 = pack halos into staging and send
                         them by copy (invoke); 0 = zero-copy: the sweep writes
                         them into messages allocated in the neighbours'

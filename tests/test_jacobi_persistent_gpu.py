@@ -38,8 +38,8 @@ def test_c1_config_matches_chare_oracle(cam):
     assert r.residual == res
     assert r.hash == O.grid_hash(ref)
     # init + end messages, and per exchange round 48 edges and 36 corners
-    # (two-deep halos: one round per two sweeps)
-    assert r.messages == expected_messages(4, 4, 100, 2)
+    # (halos eight deep by default: one round per eight sweeps)
+    assert r.messages == expected_messages(4, 4, 100, 8)
 
 
 @pytest.mark.parametrize("W,H,cr,cc,iters", [
@@ -93,12 +93,12 @@ def test_default_decomposition_uses_all_pes(cam):
 
 
 @pytest.mark.parametrize("halo_copy", [0, 1])
-@pytest.mark.parametrize("depth", [1, 2, 3])
+@pytest.mark.parametrize("depth", [1, 2, 3, 4, 5, 6, 7, 8])
 @pytest.mark.parametrize("W,H,cr,cc,iters", [
     (300, 200, 3, 5, 17), (257, 129, 4, 3, 10), (64, 64, 2, 2, 1),
     (100, 100, 5, 5, 2), (130, 70, 1, 1, 9), (500, 40, 2, 7, 5)])
 def test_halo_depths_bit_exact(cam, depth, halo_copy, W, H, cr, cc, iters):
-    # depth 1/2/3 = sweeps per exchange; halo_copy 0 = zero-copy halo messages
+    # depth = sweeps per exchange; halo_copy 0 = zero-copy halo messages
     # (allocated in the neighbours' inboxes, written by the sweep), 1 = staged
     ref, res = O.jacobi_run(W, H, iters, seed=W + H)
     with cam.Jacobi2D(W, H, iters, form="persistent", seed=W + H,
@@ -121,29 +121,39 @@ def test_fused_sweep_shapes(cam, nc, u):
         assert np.array_equal(j.grid(), ref) and r.residual == res
 
 
-@pytest.mark.parametrize("W,H,cr,cc,iters", [
-    (64, 9, 3, 2, 7),      # three-row chares (the thinnest depth 3 allows)
-    (40, 40, 8, 8, 7),     # 64 chares of 5x5: rounds of 3, 3, 1
-    (129, 257, 2, 2, 4),   # rounds of 3, 1
-    (512, 512, 4, 4, 100)])
-def test_depth3_edge_cases(cam, W, H, cr, cc, iters):
-    ref, res = O.jacobi_run(W, H, iters, seed=W * 3 + H)
-    with cam.Jacobi2D(W, H, iters, form="persistent", seed=W * 3 + H,
+def deep_cases(D):
+    return [
+        (64, 3 * D, 3, 2, 2 * D + 1),    # D-row chares (the thinnest allowed)
+        (8 * D, 8 * D, 8, 8, 2 * D + 1),  # 64 chares of D x D: rounds D, D, 1
+        (129, 257, 2, 2, D + 1),          # rounds of D, 1
+        (200, 130, 1, 1, 3 * D - 1),      # one chare: no messages
+        (512, 512, 4, 4, 100)]
+
+
+@pytest.mark.parametrize("depth", [3, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("case", range(5))
+def test_deep_halo_edge_cases(cam, depth, case):
+    W, H, cr, cc, iters = deep_cases(depth)[case]
+    ref, res = O.jacobi_run(W, H, iters, seed=W * 3 + H + depth)
+    with cam.Jacobi2D(W, H, iters, form="persistent", seed=W * 3 + H + depth,
                       chares=(cr, cc)) as j:
-        j.set_tuning(depth=3)
+        j.set_tuning(depth=depth)
         r = j.solve()
         assert np.array_equal(j.grid(), ref)
         assert r.residual == res and r.hash == O.grid_hash(ref)
-        assert r.messages == expected_messages(cr, cc, iters, 3)
+        assert r.messages == expected_messages(cr, cc, iters, depth)
 
 
-@pytest.mark.parametrize("nc,u,iters", [(1, 1, 12), (1, 2, 13), (1, 1, 14)])
-def test_fused_sweep3_shapes(cam, nc, u, iters):
+@pytest.mark.parametrize("depth,iters,band", [(3, 12, 0), (4, 13, 0), (5, 14, 0),
+                                              (6, 12, 0), (5, 11, 64), (6, 13, 7),
+                                              (4, 9, 300), (7, 15, -1),
+                                              (8, 17, 256), (8, 16, 128)])
+def test_deep_fused_sweeps_reference_init(cam, depth, iters, band):
     W, H = 1000, 600
     ref, res = O.jacobi_run(W, H, iters, seed=23, mode=1)
     with cam.Jacobi2D(W, H, iters, form="persistent", seed=23, chares=(3, 2),
                       init_mode=cam.INIT_REFERENCE) as j:
-        j.set_tuning(nc, u, depth=3)
+        j.set_tuning(1, 1, band_rows=band, depth=depth)
         r = j.solve()
         assert np.array_equal(j.grid(), ref) and r.residual == res
 
@@ -156,7 +166,10 @@ def test_unsupported_fused_shape_is_refused(cam):
         j.set_tuning(2, 1, depth=3)
         with pytest.raises(cam.ChmError):
             j.solve()
-        j.set_tuning(depth=4)
+        j.set_tuning(1, 2, depth=4)
+        with pytest.raises(cam.ChmError):
+            j.solve()
+        j.set_tuning(depth=9)
         with pytest.raises(cam.ChmError):
             j.solve()
 
