@@ -256,7 +256,7 @@ def main():
                    "band_rows": args.band or "default"},
         "hbm_gbs": round(BYTES_PER_CELL * value, 1),
         # `achieved` counts the algorithmic 16 B per cell-update; the chares
-        # sweep twice per pass (two-deep halos), so the bytes really moved
+        # sweep several times per pass (deep halos), so the bytes really moved
         # (rocprof counters, `traffic`) are fewer: their rate over the same
         # launch time is the memory system's actual load
         "hbm_traffic_gbs": (round(traffic / (kavg / 1e3) / 1e9, 1)
