@@ -47,11 +47,12 @@ __device__ inline double u0_cell(const Request& r, int64_t i, int64_t j) {
   return (r.init_mode == CHM_INIT_REFERENCE) ? 0.0 : jac::cell_u01(r.seed, i, j);
 }
 
-// Seed a tile's cells, and at the grid's edge the two halo rings beside it,
-// in both arrays.
+// Seed a tile's cells, and at the grid's edge the `depth` halo rings beside
+// it, in every array.
 __device__ void do_init(const Request& r, const TileGeom& g) {
-  const int i0 = g.r0 - (g.r0 == 1 ? 2 : 0), i1 = g.r1 + (g.r1 == r.H + 1 ? 2 : 0);
-  const int j0 = g.c0 - (g.c0 == 1 ? 2 : 0), j1 = g.c1 + (g.c1 == r.W + 1 ? 2 : 0);
+  const int d = r.depth;
+  const int i0 = g.r0 - (g.r0 == 1 ? d : 0), i1 = g.r1 + (g.r1 == r.H + 1 ? d : 0);
+  const int j0 = g.c0 - (g.c0 == 1 ? d : 0), j1 = g.c1 + (g.c1 == r.W + 1 ? d : 0);
   const int64_t cols = j1 - j0, n = (int64_t)(i1 - i0) * cols;
   for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
     const int64_t i = i0 + e / cols, j = j0 + e % cols;
@@ -144,6 +145,50 @@ __device__ void do_round2(const Request& r, const TileGeom& g, int src,
   if (RESB) add_residual(res_slot + 2, acc_b);
 }
 
+// D = 3 .. 8 sweeps of a tile from array `src` into array `dst` in one pass
+// (the deep pass of the chare program: overlapping 128-column windows, each
+// wave one contiguous share of the tile's windows x rows; kDeep - n_sw
+// leading stages copy).  RES: the last sweep's residual into res_slot[2..3].
+// Kept out of line: its register rings would otherwise weigh on the
+// engine's task loop.
+constexpr int kDeep = 8;
+template <bool RES>
+__device__ __noinline__ void do_round_deep(const double* in, double* out,
+                                           int64_t pitch, int W, int H,
+                                           int r0, int r1, int c0, int c1,
+                                           int n_sw, uint64_t* res_slot) {
+  in = uni(in);
+  out = uni(out);
+  pitch = uni(pitch);
+  W = uni(W);
+  H = uni(H);
+  r0 = uni(r0);
+  r1 = uni(r1);
+  c0 = uni(c0);
+  c1 = uni(c1);
+  n_sw = uni(n_sw);
+  constexpr int OW = jac::window_out_cols(kDeep);
+  const int rows = r1 - r0, S = (c1 - c0 + OW - 1) / OW;
+  const int WV = blockDim.x / 64, wave = threadIdx.x / 64;
+  const int64_t total = (int64_t)S * rows;
+  const int64_t per = (total + WV - 1) / WV;
+  int64_t pos = (int64_t)wave * per;
+  const int64_t end = min(total, pos + per);
+  jac::u128acc acc = {0, 0};
+  const jac::EdgeOut2 none = {nullptr, nullptr, nullptr, nullptr};
+  const jac::RingSides rs = {true, true, true, true};  // the grid's ring
+  while (pos < end) {
+    const int panel = (int)(pos / rows), rr = (int)(pos % rows);
+    const int len = (int)min((int64_t)(rows - rr), end - pos);
+    jac::sweep_panelD<kDeep, RES, false>(in, out, pitch, W, H,
+                                         c0 + panel * OW, r0 + rr,
+                                         r0 + rr + len, acc, none, rs, c1 - 1,
+                                         kDeep - n_sw);
+    pos += len;
+  }
+  if (RES) add_residual(res_slot + 2, acc);
+}
+
 // Hash of the tile's interior cells of array `src`.
 __device__ uint64_t do_hash(const Request& r, const TileGeom& g, int src) {
   const double* u = arr(r, src);
@@ -208,15 +253,15 @@ __device__ int decide_round(Prob& p, const Request& r, int rr) {
   if (aadd(&p.round_tiles[sk], 1u) != n_tiles - 1) return 0;
   // every tile has finished round rr: its residuals are complete
   if (ald(&p.state) != 0) return 0;  // stopped on the previous round
-  const int n_sw = min(2, r.max_iters - 2 * rr);
+  const int n_sw = min(r.depth, r.max_iters - r.depth * rr);
   const uint64_t alo = ald(&p.res[sk][0][0]), ahi = ald(&p.res[sk][0][1]);
   const uint64_t blo = ald(&p.res[sk][1][0]), bhi = ald(&p.res[sk][1][1]);
   int stop = 0;  // sweeps in the result when stopping
   uint64_t slo = 0, shi = 0;
   if (!tol) {
     stop = r.max_iters;
-    slo = n_sw == 2 ? blo : alo;
-    shi = n_sw == 2 ? bhi : ahi;
+    slo = n_sw >= 2 ? blo : alo;  // a pass of 2+ sweeps adds its last
+    shi = n_sw >= 2 ? bhi : ahi;  // sweep's residual to the second slot
   } else {
     uint64_t s0lo, s0hi;
     if (rr == 0) {
@@ -328,6 +373,7 @@ __device__ void admit(const EngineDesc& E, uint64_t idx, Request* s_req) {
     q.ticket = vr->ticket;
     q.slot = vr->slot;
     q.rounds = vr->rounds;
+    q.depth = vr->depth;
     *s_req = q;
     Prob& p = E.probs[q.slot & (kMaxSlots - 1)];
     p.r = q;
@@ -422,17 +468,24 @@ __device__ int find_work(const EngineDesc& E, uint32_t* task, uint64_t* req,
   }
 }
 
-// Round rr of a tile: two fused sweeps (or the last single one), with the
-// residuals the problem needs.
+// Round rr of a tile: `depth` fused sweeps (or what is left in the last
+// round), with the residuals the problem needs.
 template <int NC, int U>
 __device__ void run_round(const Request& r, const TileGeom& g, int rr,
                           Prob& p) {
   const bool tol = r.tol2 >= 0;
   const bool last = rr + 1 == r.rounds;
-  const int n_sw = min(2, r.max_iters - 2 * rr);
+  const int n_sw = min(r.depth, r.max_iters - r.depth * rr);
   uint64_t* slot = &p.res[tol ? (rr & 1) : 0][0][0];
   const int src = rr % 3, dst = (rr + 1) % 3;
-  if (n_sw == 1) {
+  if (n_sw >= 3) {  // (fixed sweep counts only: a tolerance keeps depth 2)
+    if (last)
+      do_round_deep<true>(arr(r, src), arr(r, dst), r.pitch, r.W, r.H, g.r0,
+                          g.r1, g.c0, g.c1, n_sw, slot);
+    else
+      do_round_deep<false>(arr(r, src), arr(r, dst), r.pitch, r.W, r.H, g.r0,
+                           g.r1, g.c0, g.c1, n_sw, slot);
+  } else if (n_sw == 1) {
     if (tol || last)
       do_sweep<NC, U, true>(r, g, src, dst, slot);
     else
@@ -501,8 +554,8 @@ __global__ void __launch_bounds__(T) engine_kernel(EngineDesc E) {
     } else if (kind == K_FIX) {
       do_sweep<NC, U, false>(r, g, k % 3, (k + 1) % 3, nullptr);
     } else {
-      // the result of `k` sweeps is in array ceil(k / 2) % 3
-      const uint64_t h = do_hash(r, g, ((k + 1) / 2) % 3);
+      // the result of `k` sweeps is in array ceil(k / depth) % 3
+      const uint64_t h = do_hash(r, g, ((k + r.depth - 1) / r.depth) % 3);
       if (__lane_id() == 0 && h) aadd(&p.hash, h);
     }
     drain();
@@ -513,7 +566,8 @@ __global__ void __launch_bounds__(T) engine_kernel(EngineDesc E) {
       release_agent();
       if (kind == K_ITER) {
         f = decide_round(p, r, k);
-        const uint64_t sweeps = (uint64_t)min(2, r.max_iters - 2 * k);
+        const uint64_t sweeps =
+            (uint64_t)min(r.depth, r.max_iters - r.depth * k);
         aadd(&E.ctl[CW_STATS + 3],
              sweeps * (uint64_t)(g.r1 - g.r0) * (uint64_t)(g.c1 - g.c0));
       }
@@ -564,17 +618,19 @@ __global__ void init_queues(uint64_t* cells, uint64_t n_cells, uint32_t qcap) {
     cells[i] = (uint64_t)(i & (qcap - 1)) << 32;
 }
 
-// Host-launched seeding of a whole problem: u_0 in both arrays (rows -1 ..
-// H+2), padding 0.
+// Host-launched seeding of a whole problem: u_0 in every array (rows
+// 1-depth .. H+depth), padding 0.
 __global__ void init_problem(Request r) {
-  const int64_t n = (int64_t)(r.H + 4) * r.pitch;
+  const int d = r.depth;
+  const int64_t n = (int64_t)(r.H + 2 * d) * r.pitch;
+  const int64_t o = (int64_t)(d - 1) * r.pitch;  // row 0 from the first row
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = e / r.pitch - 1, j = e % r.pitch - kColOff;
-    const double v = (j >= -1 && j <= r.W + 2) ? u0_cell(r, i, j) : 0.0;
-    r.a[e - r.pitch] = v;
-    r.b[e - r.pitch] = v;
-    if (r.c) r.c[e - r.pitch] = v;
+    const int64_t i = e / r.pitch - (d - 1), j = e % r.pitch - kColOff;
+    const double v = (j >= 1 - d && j <= r.W + d) ? u0_cell(r, i, j) : 0.0;
+    r.a[e - o] = v;
+    r.b[e - o] = v;
+    if (r.c) r.c[e - o] = v;
   }
 }
 
@@ -665,10 +721,15 @@ static Built build_request(const chm_batch_problem& pb, int block) {
   r.boundary = pb.boundary;
   r.tol2 = pb.tol > 0 ? pb.tol * pb.tol : -1.0;
   r.seed = pb.seed;
-  r.rounds = (pb.iters + 1) / 2;
-  r.pitch = jac::pitch_for_depth(r.W, 2);  // halos two deep: fused rounds
-  b.grid_bytes =
-      round_up(jac::array_elems_depth(r.W, r.H, 2) * sizeof(double), 256);
+  // rounds of eight fused sweeps for a fixed sweep count (tiles and grid at
+  // least 8 on a side), two with a tolerance (its stop rule needs each
+  // sweep's residual)
+  r.depth = (r.tol2 < 0 && std::min(std::min(r.W, r.H), r.bs) >= kDeep) ? kDeep
+                                                                         : 2;
+  r.rounds = (pb.iters + r.depth - 1) / r.depth;
+  r.pitch = jac::pitch_for_depth(r.W, r.depth);  // halos `depth` deep
+  b.grid_bytes = round_up(
+      jac::array_elems_depth(r.W, r.H, r.depth) * sizeof(double), 256);
   b.pend_bytes = round_up((size_t)r.tr * r.tc * 3 * sizeof(uint32_t), 256);
   return b;
 }
@@ -910,13 +971,13 @@ static void place_batch(std::vector<Built>& bl, char* arena, int n_arr) {
   size_t off = 0;
   for (size_t i = 0; i < bl.size(); i++) {
     Request& r = bl[i].r;
-    r.a = (double*)(arena + off) + r.pitch;  // row 0 (row -1 comes first)
+    r.a = (double*)(arena + off) + (r.depth - 1) * r.pitch;  // row 0
     off += bl[i].grid_bytes;
-    r.b = (double*)(arena + off) + r.pitch;
+    r.b = (double*)(arena + off) + (r.depth - 1) * r.pitch;
     off += bl[i].grid_bytes;
     r.c = nullptr;
     if (n_arr == 3) {
-      r.c = (double*)(arena + off) + r.pitch;
+      r.c = (double*)(arena + off) + (r.depth - 1) * r.pitch;
       off += bl[i].grid_bytes;
     }
     r.pend = (uint32_t*)(arena + off);
@@ -1530,9 +1591,10 @@ int chm_jacobi2d_server_submit(void* handle, const chm_batch_problem* pb,
     return fail(CHM_E_BUSY, "budget: %zu of %zu bytes in use", s->used,
                 s->pool_size);
   Request& r = b.r;
-  r.a = (double*)(s->pool + off) + r.pitch;  // row 0 (row -1 comes first)
-  r.b = (double*)(s->pool + off + b.grid_bytes) + r.pitch;
-  r.c = engine ? (double*)(s->pool + off + 2 * b.grid_bytes) + r.pitch : nullptr;
+  const int64_t row0 = (int64_t)(r.depth - 1) * r.pitch;  // rows 1-depth ..
+  r.a = (double*)(s->pool + off) + row0;
+  r.b = (double*)(s->pool + off + b.grid_bytes) + row0;
+  r.c = engine ? (double*)(s->pool + off + 2 * b.grid_bytes) + row0 : nullptr;
   r.pend = engine ? (uint32_t*)(s->pool + off + 3 * b.grid_bytes) : nullptr;
   r.init_on_device = 1;
   r.ticket = s->next_ticket++;

@@ -59,7 +59,9 @@ struct Request {
   uint64_t seed;
   uint64_t ticket;
   uint32_t slot;
-  int rounds;         // ceil(max_iters / 2)
+  int rounds;         // ceil(max_iters / depth)
+  int depth;          // sweeps per round = halo depth of the arrays (rows
+                      // 1-depth .. H+depth): 2 with a tolerance, else 8
 };
 
 // Device state of a problem slot (HBM, written only by the engine).

@@ -94,6 +94,7 @@ __device__ inline Request uni(const Request& s) {
   r.ticket = uni(s.ticket);
   r.slot = uni(s.slot);
   r.rounds = uni(s.rounds);
+  r.depth = uni(s.depth);
   return r;
 }
 

@@ -629,12 +629,14 @@ struct PanelW {
     if (vload) r.v = gload2(in + (int64_t)row * pitch + kColOff + c);
   }
 
-  // One sweep: row q of the next grid from three rows of the current one.
-  // CHECK = false is for rows and waves that meet no ring cell.
+  // One sweep: row q of the next grid from three rows of the current one
+  // (COPY: the row itself, a stage that does no sweep).  CHECK = false is
+  // for rows and waves that meet no ring cell.
   template <bool CHECK>
   __device__ inline void stage(const RowW& up, const RowW& cu, const RowW& dn,
-                               int q, RowW& o) const {
-    if (CHECK && ((q == 0 && ring.top) || (q == h + 1 && ring.bottom))) {
+                               int q, RowW& o, bool copy = false) const {
+    if (copy ||
+        (CHECK && ((q == 0 && ring.top) || (q == h + 1 && ring.bottom)))) {
       o = cu;
       return;
     }
@@ -675,13 +677,16 @@ struct PanelW {
 // a ring of three register slots (one row in flight), and each intermediate
 // grid keeps its three live rows in a ring of its own; ring slots are
 // static indices because the step loop is unrolled by three.  RES adds the
-// exact residual of the last sweep.
+// exact residual of the last sweep.  n_copy (0 .. D-1) leading stages only
+// copy their input, so one instantiation also does D - n_copy sweeps (the
+// halos read stay D deep).
 template <int D, bool RES, bool PACK>
 __device__ inline void sweep_panelD(const double* __restrict__ in,
                                     double* __restrict__ out, int64_t pitch,
                                     int w, int h, int c0, int r0, int r1,
                                     u128acc& acc, const EdgeOut2& eo,
-                                    RingSides rs, int cend = -1) {
+                                    RingSides rs, int cend = -1,
+                                    int n_copy = 0) {
   static_assert(D >= 3 && D <= 14, "deep fused sweep: 3 <= D <= 14");
   if (r0 >= r1) return;
   PanelW p;
@@ -713,14 +718,14 @@ __device__ inline void sweep_panelD(const double* __restrict__ in,
       if (t < n_steps) {
         const int q1 = r0 - D + 1 + t;
         p.template stage<true>(R[k], R[(k + 1) % 3], R[(k + 2) % 3], q1,
-                               S[0][k]);
+                               S[0][k], n_copy >= 1);
         if (q1 + 2 <= r1 + D - 1) p.load(R[k], q1 + 2);
 #pragma unroll
         for (int s = 2; s < D; s++)
           if (t >= 2 * (s - 1))
             p.template stage<true>(S[s - 2][(k + 1) % 3],
                                    S[s - 2][(k + 2) % 3], S[s - 2][k],
-                                   q1 - (s - 1), S[s - 1][k]);
+                                   q1 - (s - 1), S[s - 1][k], n_copy >= s);
         if (t >= 2 * (D - 1))
           p.template outrow<D, RES, PACK>(S[D - 2][(k + 1) % 3],
                                           S[D - 2][(k + 2) % 3], S[D - 2][k],
@@ -739,12 +744,13 @@ __device__ inline void sweep_panelD(const double* __restrict__ in,
       for (int k = 0; k < 3; k++) {
         const int q1 = r0 - D + 1 + t0 + k;
         p.template stage<false>(R[k], R[(k + 1) % 3], R[(k + 2) % 3], q1,
-                                S[0][k]);
+                                S[0][k], n_copy >= 1);
         p.load(R[k], q1 + 2);
 #pragma unroll
         for (int s = 2; s < D; s++)
           p.template stage<false>(S[s - 2][(k + 1) % 3], S[s - 2][(k + 2) % 3],
-                                  S[s - 2][k], q1 - (s - 1), S[s - 1][k]);
+                                  S[s - 2][k], q1 - (s - 1), S[s - 1][k],
+                                  n_copy >= s);
         p.template outrow<D, RES, PACK>(S[D - 2][(k + 1) % 3],
                                         S[D - 2][(k + 2) % 3], S[D - 2][k],
                                         q1 - (D - 1), acc, eo);
