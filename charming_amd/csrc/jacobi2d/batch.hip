@@ -189,6 +189,55 @@ __device__ __noinline__ void do_round_deep(const double* in, double* out,
   if (RES) add_residual(res_slot + 2, acc);
 }
 
+// Rounds of tolerance problems: 2 .. kDeepTol sweeps of a tile in one pass
+// (kDeepTol - n_sw leading stages copy).  RALL: the residual of every sweep
+// j = 1 .. n_sw into res_slot[2(j-1) .. 2(j-1)+1] (the stop rule needs each
+// one); else none (the repair of a stop inside a round).
+constexpr int kDeepTol = 4;
+template <bool RALL>
+__device__ __noinline__ void do_round_tol(const double* in, double* out,
+                                          int64_t pitch, int W, int H, int r0,
+                                          int r1, int c0, int c1, int n_sw,
+                                          uint64_t* res_slot) {
+  in = uni(in);
+  out = uni(out);
+  pitch = uni(pitch);
+  W = uni(W);
+  H = uni(H);
+  r0 = uni(r0);
+  r1 = uni(r1);
+  c0 = uni(c0);
+  c1 = uni(c1);
+  n_sw = uni(n_sw);
+  constexpr int D = kDeepTol;
+  constexpr int OW = jac::window_out_cols(D);
+  const int rows = r1 - r0, S = (c1 - c0 + OW - 1) / OW;
+  const int WV = blockDim.x / 64, wave = threadIdx.x / 64;
+  const int64_t total = (int64_t)S * rows;
+  const int64_t per = (total + WV - 1) / WV;
+  int64_t pos = (int64_t)wave * per;
+  const int64_t end = min(total, pos + per);
+  jac::u128acc accs[D];
+#pragma unroll
+  for (int s = 0; s < D; s++) accs[s] = {0, 0};
+  const jac::EdgeOut2 none = {nullptr, nullptr, nullptr, nullptr};
+  const jac::RingSides rs = {true, true, true, true};  // the grid's ring
+  const int n_copy = D - n_sw;
+  while (pos < end) {
+    const int panel = (int)(pos / rows), rr = (int)(pos % rows);
+    const int len = (int)min((int64_t)(rows - rr), end - pos);
+    jac::sweep_panelD<D, RALL, false, RALL>(
+        in, out, pitch, W, H, c0 + panel * OW, r0 + rr, r0 + rr + len,
+        accs[D - 1], none, rs, c1 - 1, n_copy, accs);
+    pos += len;
+  }
+  if (RALL) {
+#pragma unroll
+    for (int s = 0; s < D; s++)
+      if (s >= n_copy) add_residual(res_slot + 2 * (s - n_copy), accs[s]);
+  }
+}
+
 // Hash of the tile's interior cells of array `src`.
 __device__ uint64_t do_hash(const Request& r, const TileGeom& g, int src) {
   const double* u = arr(r, src);
@@ -254,35 +303,41 @@ __device__ int decide_round(Prob& p, const Request& r, int rr) {
   // every tile has finished round rr: its residuals are complete
   if (ald(&p.state) != 0) return 0;  // stopped on the previous round
   const int n_sw = min(r.depth, r.max_iters - r.depth * rr);
-  const uint64_t alo = ald(&p.res[sk][0][0]), ahi = ald(&p.res[sk][0][1]);
-  const uint64_t blo = ald(&p.res[sk][1][0]), bhi = ald(&p.res[sk][1][1]);
   int stop = 0;  // sweeps in the result when stopping
   uint64_t slo = 0, shi = 0;
   if (!tol) {
+    // a pass of 2+ sweeps adds its last sweep's residual to the second slot
+    const int j = n_sw >= 2 ? 1 : 0;
     stop = r.max_iters;
-    slo = n_sw >= 2 ? blo : alo;  // a pass of 2+ sweeps adds its last
-    shi = n_sw >= 2 ? bhi : ahi;  // sweep's residual to the second slot
+    slo = ald(&p.res[sk][j][0]);
+    shi = ald(&p.res[sk][j][1]);
   } else {
+    // the residual of every sweep of the round, in order
     uint64_t s0lo, s0hi;
     if (rr == 0) {
-      s0lo = alo;
-      s0hi = ahi;
-      ast(&p.s0[0], alo);
-      ast(&p.s0[1], ahi);
+      s0lo = ald(&p.res[sk][0][0]);
+      s0hi = ald(&p.res[sk][0][1]);
+      ast(&p.s0[0], s0lo);
+      ast(&p.s0[1], s0hi);
     } else {
       s0lo = ald(&p.s0[0]);
       s0hi = ald(&p.s0[1]);
     }
     const double thr = r.tol2 * u128d(s0lo, s0hi);
-    if (u128d(alo, ahi) <= thr || (last && n_sw == 1)) {
-      stop = 2 * rr + 1;
-      slo = alo;
-      shi = ahi;
-      if (n_sw == 2) ast(&p.fix_round, (uint32_t)rr);
-    } else if (n_sw == 2 && (u128d(blo, bhi) <= thr || last)) {
-      stop = 2 * rr + 2;
-      slo = blo;
-      shi = bhi;
+    for (int j = 0; j < n_sw; j++) {
+      const uint64_t lo = ald(&p.res[sk][j][0]), hi = ald(&p.res[sk][j][1]);
+      if (u128d(lo, hi) <= thr || (last && j == n_sw - 1)) {
+        stop = r.depth * rr + j + 1;
+        slo = lo;
+        shi = hi;
+        if (j + 1 < n_sw) {
+          // the round stored the grid after n_sw sweeps: the repair redoes
+          // the first j+1 from the round's intact input
+          ast(&p.fix_sweeps, (uint32_t)(j + 1));
+          ast(&p.fix_round, (uint32_t)rr);
+        }
+        break;
+      }
     }
   }
   if (stop) {
@@ -292,7 +347,7 @@ __device__ int decide_round(Prob& p, const Request& r, int rr) {
     ast(&p.state, 1u);
   } else {
     ast(&p.round_tiles[sk], 0u);
-    for (int i = 0; i < 2; i++) {
+    for (int i = 0; i < n_sw; i++) {
       ast(&p.res[sk][i][0], 0ull);
       ast(&p.res[sk][i][1], 0ull);
     }
@@ -381,9 +436,10 @@ __device__ void admit(const EngineDesc& E, uint64_t idx, Request* s_req) {
     ast(&p.state, 0u);
     ast(&p.stop_iters, 0u);
     ast(&p.fix_round, 0xffffffffu);
+    ast(&p.fix_sweeps, 0u);
     for (int i = 0; i < 2; i++) {
       ast(&p.round_tiles[i], 0u);
-      for (int j = 0; j < 2; j++) {
+      for (int j = 0; j < 8; j++) {
         ast(&p.res[i][j][0], 0ull);
         ast(&p.res[i][j][1], 0ull);
       }
@@ -478,7 +534,10 @@ __device__ void run_round(const Request& r, const TileGeom& g, int rr,
   const int n_sw = min(r.depth, r.max_iters - r.depth * rr);
   uint64_t* slot = &p.res[tol ? (rr & 1) : 0][0][0];
   const int src = rr % 3, dst = (rr + 1) % 3;
-  if (n_sw >= 3) {  // (fixed sweep counts only: a tolerance keeps depth 2)
+  if (n_sw >= 3 && tol) {
+    do_round_tol<true>(arr(r, src), arr(r, dst), r.pitch, r.W, r.H, g.r0,
+                       g.r1, g.c0, g.c1, n_sw, slot);
+  } else if (n_sw >= 3) {
     if (last)
       do_round_deep<true>(arr(r, src), arr(r, dst), r.pitch, r.W, r.H, g.r0,
                           g.r1, g.c0, g.c1, n_sw, slot);
@@ -503,7 +562,7 @@ template <int T, int NC, int U>
 __global__ void __launch_bounds__(T) engine_kernel(EngineDesc E) {
   __shared__ uint32_t s_task;
   __shared__ uint64_t s_req;
-  __shared__ int s_cmd, s_k, s_flags;
+  __shared__ int s_cmd, s_k, s_n, s_flags;
   __shared__ Request s_r;
   uint32_t rng = blockIdx.x * 2654435761u + 12345u;
   for (;;) {
@@ -540,6 +599,7 @@ __global__ void __launch_bounds__(T) engine_kernel(EngineDesc E) {
         k = (int)ald(&p.stop_iters);
       } else if (kind == K_FIX) {
         k = (int)ald(&p.fix_round);
+        s_n = (int)ald(&p.fix_sweeps);
       }
       s_k = k;
     }
@@ -552,7 +612,15 @@ __global__ void __launch_bounds__(T) engine_kernel(EngineDesc E) {
     } else if (kind == K_ITER) {
       run_round<NC, U>(r, g, k, p);
     } else if (kind == K_FIX) {
-      do_sweep<NC, U, false>(r, g, k % 3, (k + 1) % 3, nullptr);
+      // redo the first n sweeps of round k from its intact input
+      const int n = uni(s_n);
+      if (n == 1)
+        do_sweep<NC, U, false>(r, g, k % 3, (k + 1) % 3, nullptr);
+      else if (n == 2)
+        do_round2<NC, U, false, false>(r, g, k % 3, (k + 1) % 3, nullptr);
+      else
+        do_round_tol<false>(arr(r, k % 3), arr(r, (k + 1) % 3), r.pitch, r.W,
+                            r.H, g.r0, g.r1, g.c0, g.c1, n, nullptr);
     } else {
       // the result of `k` sweeps is in array ceil(k / depth) % 3
       const uint64_t h = do_hash(r, g, ((k + r.depth - 1) / r.depth) % 3);
@@ -721,11 +789,12 @@ static Built build_request(const chm_batch_problem& pb, int block) {
   r.boundary = pb.boundary;
   r.tol2 = pb.tol > 0 ? pb.tol * pb.tol : -1.0;
   r.seed = pb.seed;
-  // rounds of eight fused sweeps for a fixed sweep count (tiles and grid at
-  // least 8 on a side), two with a tolerance (its stop rule needs each
-  // sweep's residual)
-  r.depth = (r.tol2 < 0 && std::min(std::min(r.W, r.H), r.bs) >= kDeep) ? kDeep
-                                                                         : 2;
+  // rounds of eight fused sweeps for a fixed sweep count, four with a
+  // tolerance (the pass then sums every sweep's residual, in registers);
+  // tiles and grid at least that many cells on a side, else two
+  const int side = std::min(std::min(r.W, r.H), r.bs);
+  const int deep = r.tol2 < 0 ? kDeep : kDeepTol;
+  r.depth = side >= deep ? deep : 2;
   r.rounds = (pb.iters + r.depth - 1) / r.depth;
   r.pitch = jac::pitch_for_depth(r.W, r.depth);  // halos `depth` deep
   b.grid_bytes = round_up(

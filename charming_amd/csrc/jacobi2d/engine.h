@@ -70,10 +70,12 @@ struct Prob {
   uint32_t inflight;      // tasks enqueued and not finished
   uint32_t state;         // 0 running, 1 stopped, 4 redoing a sweep, 2 hashing
   uint32_t stop_iters;    // sweeps in the final result
-  uint32_t fix_round;     // round whose first sweep is redone (~0u: none)
+  uint32_t fix_round;     // round whose first fix_sweeps sweeps are redone
+                          // (~0u: none)
   uint32_t round_tiles[2];  // tiles that finished round r (slot r & 1)
-  uint32_t pad0[2];
-  uint64_t res[2][2][2];  // [r & 1][sweep 2r, 2r+1][lo, hi] residual sums
+  uint32_t fix_sweeps;
+  uint32_t pad0;
+  uint64_t res[2][8][2];  // [r & 1][sweep j of round r][lo, hi] residual sums
   uint64_t s0[2];         // residual of the first sweep
   uint64_t s_last[2];     // residual of the last sweep of the result
   uint64_t hash;

@@ -632,9 +632,13 @@ struct PanelW {
   // One sweep: row q of the next grid from three rows of the current one
   // (COPY: the row itself, a stage that does no sweep).  CHECK = false is
   // for rows and waves that meet no ring cell.
-  template <bool CHECK>
+  // RS: add the sweep's exact residual over the cells this window stores
+  // on rows [q0, q1) to acc.
+  template <bool CHECK, bool RS = false>
   __device__ inline void stage(const RowW& up, const RowW& cu, const RowW& dn,
-                               int q, RowW& o, bool copy = false) const {
+                               int q, RowW& o, bool copy = false,
+                               u128acc* acc = nullptr, int q0 = 0,
+                               int q1 = 0) const {
     if (copy ||
         (CHECK && ((q == 0 && ring.top) || (q == h + 1 && ring.bottom)))) {
       o = cu;
@@ -651,6 +655,10 @@ struct PanelW {
       if (ring_b) nb = b;
     }
     o.v = make_double2(na, nb);
+    if (RS && q >= q0 && q < q1 && vstore) {
+      if (c <= cend) acc_add(*acc, res_term(na, a));
+      if (c + 1 <= cend) acc_add(*acc, res_term(nb, b));
+    }
   }
 
   // The stored sweep: row `row` of the output, its residual against the
@@ -679,14 +687,17 @@ struct PanelW {
 // static indices because the step loop is unrolled by three.  RES adds the
 // exact residual of the last sweep.  n_copy (0 .. D-1) leading stages only
 // copy their input, so one instantiation also does D - n_copy sweeps (the
-// halos read stay D deep).
-template <int D, bool RES, bool PACK>
+// halos read stay D deep).  RALL (with RES): stage s = 1 .. D-1 also adds
+// its sweep's residual over the stored cells to accs[s-1] (acc is the last
+// stage's).
+template <int D, bool RES, bool PACK, bool RALL = false>
 __device__ inline void sweep_panelD(const double* __restrict__ in,
                                     double* __restrict__ out, int64_t pitch,
                                     int w, int h, int c0, int r0, int r1,
                                     u128acc& acc, const EdgeOut2& eo,
                                     RingSides rs, int cend = -1,
-                                    int n_copy = 0) {
+                                    int n_copy = 0, u128acc* accs = nullptr) {
+  static_assert(!RALL || RES, "per-sweep residuals include the last one");
   static_assert(D >= 3 && D <= 14, "deep fused sweep: 3 <= D <= 14");
   if (r0 >= r1) return;
   PanelW p;
@@ -717,15 +728,17 @@ __device__ inline void sweep_panelD(const double* __restrict__ in,
       const int t = t0 + k;
       if (t < n_steps) {
         const int q1 = r0 - D + 1 + t;
-        p.template stage<true>(R[k], R[(k + 1) % 3], R[(k + 2) % 3], q1,
-                               S[0][k], n_copy >= 1);
+        p.template stage<true, RALL>(R[k], R[(k + 1) % 3], R[(k + 2) % 3], q1,
+                                     S[0][k], n_copy >= 1, accs, r0, r1);
         if (q1 + 2 <= r1 + D - 1) p.load(R[k], q1 + 2);
 #pragma unroll
         for (int s = 2; s < D; s++)
           if (t >= 2 * (s - 1))
-            p.template stage<true>(S[s - 2][(k + 1) % 3],
-                                   S[s - 2][(k + 2) % 3], S[s - 2][k],
-                                   q1 - (s - 1), S[s - 1][k], n_copy >= s);
+            p.template stage<true, RALL>(S[s - 2][(k + 1) % 3],
+                                         S[s - 2][(k + 2) % 3], S[s - 2][k],
+                                         q1 - (s - 1), S[s - 1][k], n_copy >= s,
+                                         RALL ? accs + (s - 1) : nullptr, r0,
+                                         r1);
         if (t >= 2 * (D - 1))
           p.template outrow<D, RES, PACK>(S[D - 2][(k + 1) % 3],
                                           S[D - 2][(k + 2) % 3], S[D - 2][k],
@@ -743,14 +756,15 @@ __device__ inline void sweep_panelD(const double* __restrict__ in,
 #pragma unroll
       for (int k = 0; k < 3; k++) {
         const int q1 = r0 - D + 1 + t0 + k;
-        p.template stage<false>(R[k], R[(k + 1) % 3], R[(k + 2) % 3], q1,
-                                S[0][k], n_copy >= 1);
+        p.template stage<false, RALL>(R[k], R[(k + 1) % 3], R[(k + 2) % 3],
+                                      q1, S[0][k], n_copy >= 1, accs, r0, r1);
         p.load(R[k], q1 + 2);
 #pragma unroll
         for (int s = 2; s < D; s++)
-          p.template stage<false>(S[s - 2][(k + 1) % 3], S[s - 2][(k + 2) % 3],
-                                  S[s - 2][k], q1 - (s - 1), S[s - 1][k],
-                                  n_copy >= s);
+          p.template stage<false, RALL>(
+              S[s - 2][(k + 1) % 3], S[s - 2][(k + 2) % 3], S[s - 2][k],
+              q1 - (s - 1), S[s - 1][k], n_copy >= s,
+              RALL ? accs + (s - 1) : nullptr, r0, r1);
         p.template outrow<D, RES, PACK>(S[D - 2][(k + 1) % 3],
                                         S[D - 2][(k + 2) % 3], S[D - 2][k],
                                         q1 - (D - 1), acc, eo);

@@ -3,10 +3,11 @@ forms), run under random interleavings and checked against the oracle.
 
 The model follows the device code of charming_amd/csrc/jacobi2d/batch.hip
 step for step - per-tile counters for the next even/odd round, re-arming at
-task start, up to eight neighbours, the stop-decision gate, the repair of a
-stop after the first sweep of a fused round, the in-flight count and the
-state machine - while the stencil work is done with numpy on the shared
-arrays.  Every task is split into a read phase and a write phase with other
+task start, up to eight neighbours, rounds of `depth` fused sweeps (eight
+for a fixed sweep count, four with a tolerance, two for small tiles), the
+stop-decision gate, the per-sweep residuals, the repair of a stop inside a
+fused round, the in-flight count and the state machine - while the stencil
+work is done with numpy on the shared arrays.  Every task is split into a read phase and a write phase with other
 tasks free to run in between, so two tasks the protocol lets overlap do
 overlap.  If the protocol ever let a tile read cells a neighbour had not
 finished, or overwrite cells a neighbour still needed, the result would
@@ -50,13 +51,21 @@ def sweep_region(src, r0, r1, c0, c1, H, W):
     return out
 
 
+def depth_for(W, H, bs, tol2):
+    """Sweeps per round (build_request): 8 for a fixed sweep count, 4 with a
+    tolerance, 2 when the tile or grid is thinner than that."""
+    deep = 8 if tol2 < 0 else 4
+    return deep if min(W, H, bs) >= deep else 2
+
+
 class Problem:
     def __init__(self, W, H, bs, iters, tol, seed):
         self.W, self.H, self.bs, self.iters = W, H, bs, iters
         self.tol2 = tol * tol if tol > 0 else -1.0
         self.tr, self.tc = -(-H // bs), -(-W // bs)
         self.n = self.tr * self.tc
-        self.rounds = (iters + 1) // 2
+        self.depth = depth_for(W, H, bs, self.tol2)
+        self.rounds = -(-iters // self.depth)
         g = O.init_grid(W, H, seed)
         self.A = [g.copy(), g.copy(), g.copy()]
         self.pend = []
@@ -66,8 +75,9 @@ class Problem:
         self.inflight = self.n
         self.state = 0
         self.fix_round = None
+        self.fix_sweeps = 0
         self.round_tiles = [0, 0]
-        self.res = [[0, 0], [0, 0]]
+        self.res = [[0] * 8, [0] * 8]
         self.s0 = 0
         self.stop_iters = 0
         self.s_last = 0
@@ -114,42 +124,58 @@ class Engine:
             return k
         return p.fix_round if kind == "fix" else None
 
+    @staticmethod
+    def fused(p, src, n_sw, r0, r1, c0, c1):
+        """n_sw sweeps of the tile in one pass: sweep s over the tile grown
+        by n_sw - s cells (what the pass reads), every sweep's residual over
+        the tile's own cells."""
+        big = src.copy()
+        res = []
+        for s in range(1, n_sw + 1):
+            e = n_sw - s
+            a0, a1 = max(r0 - e, 0), min(r1 + e, p.H + 2)
+            b0, b1 = max(c0 - e, 0), min(c1 + e, p.W + 2)
+            new = sweep_region(big, a0, a1, b0, b1, p.H, p.W)
+            tile = new[r0 - a0:r1 - a0, c0 - b0:c1 - b0]
+            res.append(res_sum(tile, big[r0:r1, c0:c1]))
+            big[a0:a1, b0:b1] = new
+        return big[r0:r1, c0:c1].copy(), res
+
     # ---- the read phase: at the task's start or just before its end ----
     def read(self, t, k):
         kind, pi, u = t
         p = self.probs[pi]
         r0, r1, c0, c1 = p.geom(u)
         if kind == "iter":
-            n_sw = min(2, p.iters - 2 * k)
-            src = p.A[k % 3]
+            n_sw = min(p.depth, p.iters - p.depth * k)
             tol, last = p.tol2 >= 0, k + 1 == p.rounds
-            if n_sw == 1:
-                new = sweep_region(src, r0, r1, c0, c1, p.H, p.W)
-                ra = res_sum(new, src[r0:r1, c0:c1]) if (tol or last) else 0
-                rb = 0
+            new, res = self.fused(p, p.A[k % 3], n_sw, r0, r1, c0, c1)
+            # what the device adds: every sweep's residual with a tolerance;
+            # on the last round of a fixed count the last one, into the
+            # second slot for a pass of two or more sweeps
+            if tol:
+                add = res
+            elif last:
+                add = [res[-1]] if n_sw == 1 else [0, res[-1]]
             else:
-                mid = sweep_region(src, r0 - 1, r1 + 1, c0 - 1, c1 + 1, p.H, p.W)
-                big = src.copy()
-                big[r0 - 1:r1 + 1, c0 - 1:c1 + 1] = mid
-                new = sweep_region(big, r0, r1, c0, c1, p.H, p.W)
-                ra = res_sum(mid[1:-1, 1:-1], src[r0:r1, c0:c1]) if tol else 0
-                rb = res_sum(new, mid[1:-1, 1:-1]) if (tol or last) else 0
-            return (k, new, ra, rb)
+                add = []
+            return (k, new, add)
         if kind == "fix":
-            return (k, sweep_region(p.A[k % 3], r0, r1, c0, c1, p.H, p.W), 0, 0)
-        res_arr = ((p.stop_iters + 1) // 2) % 3
-        return (None, O.grid_hash(p.A[res_arr], r0, r1, c0, c1), 0, 0)
+            new, _ = self.fused(p, p.A[k % 3], p.fix_sweeps, r0, r1, c0, c1)
+            return (k, new, [])
+        res_arr = ((p.stop_iters + p.depth - 1) // p.depth) % 3
+        return (None, O.grid_hash(p.A[res_arr], r0, r1, c0, c1), [])
 
     # ---- task end: the write phase and the completion logic ----
     def finish(self, t, data):
         kind, pi, u = t
         p = self.probs[pi]
         r0, r1, c0, c1 = p.geom(u)
-        k, new, ra, rb = data
+        k, new, add = data
         if kind == "iter":
             p.A[(k + 1) % 3][r0:r1, c0:c1] = new
             p.pend[u][2] = k + 1
-            flags = self.complete_round(p, pi, u, k, ra, rb)
+            flags = self.complete_round(p, pi, u, k, add)
             if flags == "gate" and k + 2 < p.rounds:
                 for v in range(p.n):
                     p.pend[v][k & 1] -= 1
@@ -162,8 +188,8 @@ class Engine:
             p.hash = (p.hash + new) % 2 ** 64
         self.finish_task(p, pi)
 
-    def complete_round(self, p, pi, u, k, ra, rb):
-        flags = self.decide(p, k, ra, rb)
+    def complete_round(self, p, pi, u, k, add):
+        flags = self.decide(p, k, add)
         if k + 1 < p.rounds:
             for v in [u] + p.nbrs(u):
                 p.pend[v][(k + 1) & 1] -= 1
@@ -173,38 +199,37 @@ class Engine:
                     self.push(("iter", pi, v))
         return flags
 
-    def decide(self, p, k, ra, rb):
+    def decide(self, p, k, add):
         tol, last = p.tol2 >= 0, k + 1 == p.rounds
         if not tol and not last:
             return None
         sk = (k & 1) if tol else 0
-        p.res[sk][0] += ra
-        p.res[sk][1] += rb
+        for j, v in enumerate(add):
+            p.res[sk][j] += v
         p.round_tiles[sk] += 1
         if p.round_tiles[sk] != p.n:
             return None
         if p.state != 0:
             return None  # a stop was decided on the previous round
-        n_sw = min(2, p.iters - 2 * k)
-        a, b = p.res[sk]
+        n_sw = min(p.depth, p.iters - p.depth * k)
         stop = 0
         if not tol:
-            stop, s = p.iters, (b if n_sw == 2 else a)
+            stop, s = p.iters, p.res[sk][1 if n_sw >= 2 else 0]
         else:
             if k == 0:
-                p.s0 = a
+                p.s0 = p.res[sk][0]
             thr = p.tol2 * u128d(p.s0)
-            if u128d(a) <= thr or (last and n_sw == 1):
-                stop, s = 2 * k + 1, a
-                if n_sw == 2:
-                    p.fix_round = k
-            elif n_sw == 2 and (u128d(b) <= thr or last):
-                stop, s = 2 * k + 2, b
+            for j in range(n_sw):
+                if u128d(p.res[sk][j]) <= thr or (last and j == n_sw - 1):
+                    stop, s = p.depth * k + j + 1, p.res[sk][j]
+                    if j + 1 < n_sw:
+                        p.fix_round, p.fix_sweeps = k, j + 1
+                    break
         if stop:
             p.s_last, p.stop_iters, p.state = s, stop, 1
             return None
         p.round_tiles[sk] = 0
-        p.res[sk] = [0, 0]
+        p.res[sk] = [0] * 8
         return "gate"
 
     def finish_task(self, p, pi):
@@ -242,7 +267,9 @@ class Engine:
 CASES = [  # W, H, tile, sweeps, tol
     (40, 30, 16, 9, 0.0), (33, 17, 16, 6, 0.0), (20, 20, 8, 1, 0.0),
     (50, 40, 16, 400, 3e-2), (24, 24, 8, 400, 0.5), (30, 18, 8, 7, 1e-9),
-    (17, 33, 16, 400, 1e-2), (12, 12, 4, 2, 0.0)]
+    (17, 33, 16, 400, 1e-2), (12, 12, 4, 2, 0.0), (40, 40, 16, 29, 0.0),
+    (26, 19, 8, 400, 2e-2), (45, 21, 8, 400, 7e-3), (20, 20, 4, 15, 0.0),
+    (24, 24, 8, 400, 5e-2), (30, 22, 8, 400, 0.3)]
 
 
 @pytest.mark.parametrize("seed", [1, 2, 3, 4, 5, 6])

@@ -112,6 +112,28 @@ def test_tolerance_stop_matches_oracle(cam):
     assert any(0 < it < 400 for it in out["iters"][:-1])
 
 
+def test_tolerance_stop_at_every_sweep_of_a_round(cam):
+    # tolerance rounds fuse four sweeps and sum each one's residual; a stop
+    # on sweep 1, 2 or 3 of a round is repaired by redoing that many sweeps
+    # from the round's input, a stop on sweep 4 keeps the round's output;
+    # caps that leave a short last round (6, 7, 9 sweeps) too
+    tols = [0.5 * 0.7 ** k for k in range(16)]
+    probs = [{"width": 48, "height": 40, "iters": 400, "tol": t, "seed": 31,
+              "block": 16} for t in tols]
+    probs += [{"width": 40, "height": 44, "iters": it, "tol": 1e-12, "seed": it,
+               "block": 16} for it in (6, 7, 9)]
+    out = cam.batch_run(probs)
+    stops = set()
+    for i, p in enumerate(probs):
+        g, n, s_last, s0 = O.jacobi_solve_tol(p["width"], p["height"], p["iters"],
+                                              p["tol"], seed=p["seed"])
+        assert out["iters"][i] == n, (i, p)
+        assert out["hashes"][i] == O.grid_hash(g), (i, p)
+        assert out["residuals"][i] == s_last, (i, p)
+        stops.add(n % 4)
+    assert stops == {0, 1, 2, 3}
+
+
 def test_plain_and_graph_forms_match_engine(cam):
     probs = mixed_problems(12, 4, side_hi=500, it_hi=25)
     for p in probs:
