@@ -102,6 +102,24 @@ __device__ void add_residual(uint64_t* slot, jac::u128acc acc) {
   __syncthreads();
 }
 
+// Workgroup-collective: the FP64 sum of every lane's v added to *slot
+// with one atomic per task.
+__device__ void add_residual_d(double* slot, double v) {
+  __shared__ double part_d[16];
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int wave = threadIdx.x / 64, n_waves = blockDim.x / 64;
+  if (__lane_id() == 0) part_d[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < n_waves; w++) t += part_d[w];
+    if (t != 0.0)
+      __hip_atomic_fetch_add(slot, t, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+}
+
 // One sweep of a tile from array `src` into array `dst`.
 template <int NC, int U, bool RES>
 __device__ void do_sweep(const Request& r, const TileGeom& g, int src, int dst,
@@ -189,16 +207,19 @@ __device__ __noinline__ void do_round_deep(const double* in, double* out,
   if (RES) add_residual(res_slot + 2, acc);
 }
 
-// Rounds of tolerance problems: 2 .. kDeepTol sweeps of a tile in one pass
-// (kDeepTol - n_sw leading stages copy).  RALL: the residual of every sweep
-// j = 1 .. n_sw into res_slot[2(j-1) .. 2(j-1)+1] (the stop rule needs each
-// one); else none (the repair of a stop inside a round).
-constexpr int kDeepTol = 4;
-template <bool RALL>
+// Rounds of tolerance problems: 3 .. kDeepTol sweeps of a tile in one pass
+// (kDeepTol - n_sw leading stages copy).  MODE 1: the exact residual of
+// every sweep j = 1 .. n_sw into res_slot[2(j-1) .. 2(j-1)+1] (the stop
+// rule needs each one); MODE 2: the FP64 sum of its squared changes into
+// resd_slot[j-1] instead (the stop filter); MODE 0: none (the repair of a
+// stop inside a round).
+constexpr int kDeepTol = 6;
+template <int MODE>
 __device__ __noinline__ void do_round_tol(const double* in, double* out,
                                           int64_t pitch, int W, int H, int r0,
                                           int r1, int c0, int c1, int n_sw,
-                                          uint64_t* res_slot) {
+                                          uint64_t* res_slot,
+                                          double* resd_slot) {
   in = uni(in);
   out = uni(out);
   pitch = uni(pitch);
@@ -218,23 +239,31 @@ __device__ __noinline__ void do_round_tol(const double* in, double* out,
   int64_t pos = (int64_t)wave * per;
   const int64_t end = min(total, pos + per);
   jac::u128acc accs[D];
+  double accd[D];
 #pragma unroll
-  for (int s = 0; s < D; s++) accs[s] = {0, 0};
+  for (int s = 0; s < D; s++) {
+    accs[s] = {0, 0};
+    accd[s] = 0.0;
+  }
   const jac::EdgeOut2 none = {nullptr, nullptr, nullptr, nullptr};
   const jac::RingSides rs = {true, true, true, true};  // the grid's ring
   const int n_copy = D - n_sw;
   while (pos < end) {
     const int panel = (int)(pos / rows), rr = (int)(pos % rows);
     const int len = (int)min((int64_t)(rows - rr), end - pos);
-    jac::sweep_panelD<D, RALL, false, RALL>(
+    jac::sweep_panelD<D, MODE == 1, false, MODE>(
         in, out, pitch, W, H, c0 + panel * OW, r0 + rr, r0 + rr + len,
-        accs[D - 1], none, rs, c1 - 1, n_copy, accs);
+        accs[D - 1], none, rs, c1 - 1, n_copy, accs, accd);
     pos += len;
   }
-  if (RALL) {
+  if (MODE == 1) {
 #pragma unroll
     for (int s = 0; s < D; s++)
       if (s >= n_copy) add_residual(res_slot + 2 * (s - n_copy), accs[s]);
+  } else if (MODE == 2) {
+#pragma unroll
+    for (int s = 0; s < D; s++)
+      if (s >= n_copy) add_residual_d(resd_slot + (s - n_copy), accd[s]);
   }
 }
 
@@ -254,7 +283,15 @@ __device__ uint64_t do_hash(const Request& r, const TileGeom& g, int src) {
 // ---------------------------------------------------------------------------
 // completion logic
 // ---------------------------------------------------------------------------
-enum : int { F_GATE = 1, F_FINALIZE = 2, F_PUBLISH = 4, F_FIX = 8 };
+enum : int {
+  F_GATE = 1,      // round rr+2 of every tile may go
+  F_FINALIZE = 2,
+  F_PUBLISH = 4,
+  F_FIX = 8,
+  F_RESID = 16,    // recompute round rr with exact residuals (every tile);
+                   // this tile's release of round rr+1 waits for it
+  F_HELD = 32      // the held tile's release of round rr+1 may go
+};
 
 __host__ __device__ inline double u128d(uint64_t lo, uint64_t hi) {
   return ldexp((double)hi, 64) + (double)lo;
@@ -279,29 +316,42 @@ __device__ void release_deps(const EngineDesc& E, Prob& p, const Request& r,
   }
 }
 
-// Thread 0: count the tile in round rr; the last tile of a round whose
-// residual matters decides whether the problem stops.  Returns F_GATE when
-// the decision lets round rr+2 of every tile go (its gate is then dropped
-// by the whole workgroup).
-//
-// With a tolerance, round rr+2 waits for the decision on round rr, and the
-// decision comes before the release of round rr+1 (engine_kernel), so
-// decisions are made in order and round rr's input (array rr % 3) is intact
-// when it is decided even if round rr+1 is running.  Stop after the first
-// sweep k with S_k <= tol^2 * S_0 (the rule and the doubles of the
-// monolithic tolerance solve): if k is the second sweep of the round, its
-// output is the result; if it is the first, whose output the fused round
-// never stored, every tile redoes that sweep alone from the intact input
-// (K_FIX tasks).
-__device__ int decide_round(Prob& p, const Request& r, int rr) {
+// Rounds of a tolerance problem whose residuals are FP64 sums (a filter):
+// every round but the first (it sets S_0) and the last (the cap's residual
+// is the result's) of a problem with deep rounds.
+__device__ inline bool filtered_round(const Request& r, int rr) {
+  return r.tol2 >= 0 && r.depth >= 3 && rr > 0 && rr + 1 < r.rounds;
+}
+
+// Thread 0: whether a filtered round may hold the stop.  The exact residual
+// of sweep j is S_j = sum floor(fl(d^2) 2^56) over the cells (saturating at
+// 2^64 - 1), so S_j >= 2^56 T_j - N with T_j = sum fl(d^2) when no term
+// saturates, and the FP64 sum A_j of the same fl(d^2) (N <= 2^30 terms, any
+// order) is within a factor 1 + 2^-23 of T_j.  A_j 2^56 (1 - 2^-20) - N >
+// thr (1 + 2^-20) therefore proves S_j > thr; a saturated term makes S_j >=
+// 2^64 - 1, more than any thr below 2^62.
+__device__ bool filter_may_stop(Prob& p, const Request& r, int sk, int n_sw) {
+  const double thr = r.tol2 * u128d(ald(&p.s0[0]), ald(&p.s0[1]));
+  if (!(thr < 0x1.0p62)) return true;
+  const double n = (double)r.W * (double)r.H;
+  for (int j = 0; j < n_sw; j++) {
+    const double a = __longlong_as_double((long long)ald((uint64_t*)&p.resd[sk][j]));
+    if (!(a * 0x1.0p56 * (1.0 - 0x1.0p-20) - n > thr * (1.0 + 0x1.0p-20)))
+      return true;
+  }
+  return false;
+}
+
+// Thread 0: the decision on round rr from exact residuals (state 0, or 5
+// when they were recomputed).  Stop after the first sweep k with S_k <=
+// tol^2 * S_0 (the rule and the doubles of the monolithic tolerance solve):
+// if k is the round's last sweep its output is the result; else the round
+// stored a later grid, and every tile redoes the round's first sweeps up to
+// k from the intact input (K_FIX tasks).  Returns F_GATE when round rr+2
+// may go.
+__device__ int decide_exact(Prob& p, const Request& r, int rr, int sk) {
   const bool tol = r.tol2 >= 0;
   const bool last = rr + 1 == r.rounds;
-  if (!tol && !last) return 0;
-  const uint32_t n_tiles = (uint32_t)(r.tr * r.tc);
-  const int sk = tol ? (rr & 1) : 0;
-  if (aadd(&p.round_tiles[sk], 1u) != n_tiles - 1) return 0;
-  // every tile has finished round rr: its residuals are complete
-  if (ald(&p.state) != 0) return 0;  // stopped on the previous round
   const int n_sw = min(r.depth, r.max_iters - r.depth * rr);
   int stop = 0;  // sweeps in the result when stopping
   uint64_t slo = 0, shi = 0;
@@ -350,10 +400,51 @@ __device__ int decide_round(Prob& p, const Request& r, int rr) {
     for (int i = 0; i < n_sw; i++) {
       ast(&p.res[sk][i][0], 0ull);
       ast(&p.res[sk][i][1], 0ull);
+      ast((uint64_t*)&p.resd[sk][i], 0ull);
     }
+    ast(&p.state, 0u);
   }
   release_agent();
   return (!stop && rr + 2 < r.rounds) ? F_GATE : 0;
+}
+
+// Thread 0: count the tile in round rr; the last tile of a round whose
+// residual matters decides whether the problem stops.  Returns F_GATE when
+// the decision lets round rr+2 of every tile go (its gate is then dropped
+// by the whole workgroup), F_RESID when a filtered round may hold the stop:
+// every tile then recomputes it with exact residuals (its input is intact:
+// round rr+2 waits for this decision) and the decision is made on those,
+// while this tile's release of round rr+1 waits (so no later round can be
+// decided first).
+//
+// With a tolerance, round rr+2 waits for the decision on round rr, and the
+// decision comes before the release of round rr+1 (engine_kernel), so
+// decisions are made in order and round rr's input (array rr % 3) is intact
+// when it is decided even if round rr+1 is running.
+__device__ int decide_round(Prob& p, const Request& r, int rr) {
+  const bool tol = r.tol2 >= 0;
+  const bool last = rr + 1 == r.rounds;
+  if (!tol && !last) return 0;
+  const uint32_t n_tiles = (uint32_t)(r.tr * r.tc);
+  const int sk = tol ? (rr & 1) : 0;
+  if (aadd(&p.round_tiles[sk], 1u) != n_tiles - 1) return 0;
+  // every tile has finished round rr: its residuals are complete
+  if (ald(&p.state) != 0) return 0;  // stopped on the previous round
+  if (filtered_round(r, rr)) {
+    if (filter_may_stop(p, r, sk, r.depth)) {
+      ast(&p.resid_round, (uint32_t)rr);
+      ast(&p.resid_tiles, 0u);
+      ast(&p.state, 5u);
+      aadd(&p.inflight, n_tiles);  // the recompute tasks
+      release_agent();
+      return F_RESID;
+    }
+    ast(&p.round_tiles[sk], 0u);
+    for (int i = 0; i < r.depth; i++) ast((uint64_t*)&p.resd[sk][i], 0ull);
+    release_agent();
+    return rr + 2 < r.rounds ? F_GATE : 0;
+  }
+  return decide_exact(p, r, rr, sk);
 }
 
 // Thread 0: a task of the problem is finished.  The last task of a stopped
@@ -442,8 +533,12 @@ __device__ void admit(const EngineDesc& E, uint64_t idx, Request* s_req) {
       for (int j = 0; j < 8; j++) {
         ast(&p.res[i][j][0], 0ull);
         ast(&p.res[i][j][1], 0ull);
+        ast((uint64_t*)&p.resd[i][j], 0ull);
       }
     }
+    ast(&p.resid_round, 0xffffffffu);
+    ast(&p.resid_tiles, 0u);
+    ast(&p.held_tile, 0u);
     ast(&p.s0[0], 0ull);
     ast(&p.s0[1], 0ull);
     ast(&p.s_last[0], 0ull);
@@ -534,9 +629,12 @@ __device__ void run_round(const Request& r, const TileGeom& g, int rr,
   const int n_sw = min(r.depth, r.max_iters - r.depth * rr);
   uint64_t* slot = &p.res[tol ? (rr & 1) : 0][0][0];
   const int src = rr % 3, dst = (rr + 1) % 3;
-  if (n_sw >= 3 && tol) {
-    do_round_tol<true>(arr(r, src), arr(r, dst), r.pitch, r.W, r.H, g.r0,
-                       g.r1, g.c0, g.c1, n_sw, slot);
+  if (n_sw >= 3 && tol && filtered_round(r, rr)) {
+    do_round_tol<2>(arr(r, src), arr(r, dst), r.pitch, r.W, r.H, g.r0, g.r1,
+                    g.c0, g.c1, n_sw, nullptr, &p.resd[rr & 1][0]);
+  } else if (n_sw >= 3 && tol) {
+    do_round_tol<1>(arr(r, src), arr(r, dst), r.pitch, r.W, r.H, g.r0, g.r1,
+                    g.c0, g.c1, n_sw, slot, nullptr);
   } else if (n_sw >= 3) {
     if (last)
       do_round_deep<true>(arr(r, src), arr(r, dst), r.pitch, r.W, r.H, g.r0,
@@ -598,8 +696,15 @@ __global__ void __launch_bounds__(T) engine_kernel(EngineDesc E) {
       } else if (kind == K_HASH) {
         k = (int)ald(&p.stop_iters);
       } else if (kind == K_FIX) {
-        k = (int)ald(&p.fix_round);
-        s_n = (int)ald(&p.fix_sweeps);
+        // a repair (state 4) or, in state 5, the exact recompute of a
+        // filtered round's residuals (n = -1)
+        if (ald(&p.state) == 5) {
+          k = (int)ald(&p.resid_round);
+          s_n = -1;
+        } else {
+          k = (int)ald(&p.fix_round);
+          s_n = (int)ald(&p.fix_sweeps);
+        }
       }
       s_k = k;
     }
@@ -611,6 +716,13 @@ __global__ void __launch_bounds__(T) engine_kernel(EngineDesc E) {
       do_init(r, g);
     } else if (kind == K_ITER) {
       run_round<NC, U>(r, g, k, p);
+    } else if (kind == K_FIX && uni(s_n) < 0) {
+      // round k again from its intact input, now with exact residuals (the
+      // grid it writes is bit-identical to what the round wrote)
+      do_round_tol<1>(arr(r, k % 3), arr(r, (k + 1) % 3), r.pitch, r.W, r.H,
+                      g.r0, g.r1, g.c0, g.c1,
+                      min(r.depth, r.max_iters - r.depth * k),
+                      &p.res[k & 1][0][0], nullptr);
     } else if (kind == K_FIX) {
       // redo the first n sweeps of round k from its intact input
       const int n = uni(s_n);
@@ -619,8 +731,8 @@ __global__ void __launch_bounds__(T) engine_kernel(EngineDesc E) {
       else if (n == 2)
         do_round2<NC, U, false, false>(r, g, k % 3, (k + 1) % 3, nullptr);
       else
-        do_round_tol<false>(arr(r, k % 3), arr(r, (k + 1) % 3), r.pitch, r.W,
-                            r.H, g.r0, g.r1, g.c0, g.c1, n, nullptr);
+        do_round_tol<0>(arr(r, k % 3), arr(r, (k + 1) % 3), r.pitch, r.W,
+                        r.H, g.r0, g.r1, g.c0, g.c1, n, nullptr, nullptr);
     } else {
       // the result of `k` sweeps is in array ceil(k / depth) % 3
       const uint64_t h = do_hash(r, g, ((k + r.depth - 1) / r.depth) % 3);
@@ -628,30 +740,55 @@ __global__ void __launch_bounds__(T) engine_kernel(EngineDesc E) {
     }
     drain();
     __syncthreads();
+    const uint32_t n_tiles = (uint32_t)(r.tr * r.tc);
+    __shared__ uint32_t s_held;
     if (threadIdx.x == 0) {
       int f = 0;
       if (kind == K_ITER) ast(&r.pend[3 * tile + 2], (uint32_t)(k + 1));
       release_agent();
       if (kind == K_ITER) {
         f = decide_round(p, r, k);
+        if (f & F_RESID) {
+          ast(&p.held_tile, tile);
+          release_agent();
+        }
         const uint64_t sweeps =
             (uint64_t)min(r.depth, r.max_iters - r.depth * k);
         aadd(&E.ctl[CW_STATS + 3],
              sweeps * (uint64_t)(g.r1 - g.r0) * (uint64_t)(g.c1 - g.c0));
+      } else if (kind == K_FIX && s_n < 0) {
+        // the last tile of an exact recompute decides the round; without a
+        // stop the held tile's release of round k+1 goes too
+        if (aadd(&p.resid_tiles, 1u) == n_tiles - 1) {
+          acquire_agent();
+          f = decide_exact(p, r, k, k & 1);
+          if (ald(&p.state) == 0) {
+            f |= F_HELD;
+            s_held = ald(&p.held_tile);
+          }
+        }
       }
       aadd(&E.ctl[CW_STATS + 0], 1ull);
       s_flags = f;
     }
     __syncthreads();
+    const int fl = uni(s_flags);
     // after the decision: release the tile's and its neighbours' next round
+    // (held back while its round is recomputed)
     if (threadIdx.x < 64) {
       const int word = kind == K_INIT ? 0
                        : (kind == K_ITER && k + 1 < r.rounds) ? ((k + 1) & 1)
                                                               : -1;
-      if (word >= 0) release_deps(E, p, r, slot, tile, word);
+      if (word >= 0 && !(fl & F_RESID)) release_deps(E, p, r, slot, tile, word);
+      if (fl & F_HELD)
+        release_deps(E, p, r, slot, uni(s_held), (k + 1) & 1);
     }
-    const uint32_t n_tiles = (uint32_t)(r.tr * r.tc);
-    if (uni(s_flags) & F_GATE) {
+    if (fl & F_RESID) {  // (counted in flight by the decision)
+      for (uint32_t u = threadIdx.x; u < n_tiles; u += blockDim.x)
+        push_task(E, task_word(K_FIX, slot, u),
+                  (int)((blockIdx.x + u) % (uint32_t)E.n_wg));
+    }
+    if (fl & F_GATE) {
       // the decision on round k lets round k+2 (parity k & 1) go
       if (k + 2 < r.rounds) {
         for (uint32_t u = threadIdx.x; u < n_tiles; u += blockDim.x)
@@ -789,7 +926,7 @@ static Built build_request(const chm_batch_problem& pb, int block) {
   r.boundary = pb.boundary;
   r.tol2 = pb.tol > 0 ? pb.tol * pb.tol : -1.0;
   r.seed = pb.seed;
-  // rounds of eight fused sweeps for a fixed sweep count, four with a
+  // rounds of eight fused sweeps for a fixed sweep count, six with a
   // tolerance (the pass then sums every sweep's residual, in registers);
   // tiles and grid at least that many cells on a side, else two
   const int side = std::min(std::min(r.W, r.H), r.bs);

@@ -68,7 +68,8 @@ struct Request {
 struct Prob {
   Request r;
   uint32_t inflight;      // tasks enqueued and not finished
-  uint32_t state;         // 0 running, 1 stopped, 4 redoing a sweep, 2 hashing
+  uint32_t state;         // 0 running, 1 stopped, 4 redoing sweeps, 2
+                          // hashing, 5 recomputing a round's residuals
   uint32_t stop_iters;    // sweeps in the final result
   uint32_t fix_round;     // round whose first fix_sweeps sweeps are redone
                           // (~0u: none)
@@ -76,6 +77,13 @@ struct Prob {
   uint32_t fix_sweeps;
   uint32_t pad0;
   uint64_t res[2][8][2];  // [r & 1][sweep j of round r][lo, hi] residual sums
+  double resd[2][8];      // [r & 1][sweep j]: FP64 sums of squared changes
+                          // (the stop filter of rounds between the first
+                          // and the last)
+  uint32_t resid_round;   // round recomputed with exact residuals (state 5)
+  uint32_t resid_tiles;   // tiles done with that
+  uint32_t held_tile;     // tile whose release of round resid_round+1 waits
+  uint32_t pad1;
   uint64_t s0[2];         // residual of the first sweep
   uint64_t s_last[2];     // residual of the last sweep of the result
   uint64_t hash;

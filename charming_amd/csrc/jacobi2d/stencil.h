@@ -564,10 +564,11 @@ __device__ inline void store_pair(gdouble* orow, int c, int cend, int w, int h,
 
 // Deep fusion works on overlapping windows: a wave holds 128 columns of a
 // row (lane l: the pair at w0+2l, w0+2l+1), each sweep leaves one more
-// column at either end without a valid neighbour, and after D sweeps the
-// middle 128-2H columns (H = D rounded up to even, so windows stay 16-byte
-// aligned) are stored.  Neighbouring windows overlap by 2H columns; the
-// lane shifts need no boundary fix-ups.
+// column at either end without a valid neighbour, and after D <= 8 sweeps
+// the middle 112 columns are stored: margins of H = 8 columns whatever D
+// is, so a window's output is seven whole 128-byte lines (column 1 starts a
+// line) and every store is line-aligned.  Neighbouring windows overlap by
+// 2H columns; the lane shifts need no boundary fix-ups.
 struct RowW {
   double2 v;
 };
@@ -586,9 +587,11 @@ __device__ inline double lane_shl1z(double v) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// Output columns per window for D fused sweeps.
+// Columns of a window's margin, and its output columns, for D <= 8 fused
+// sweeps.
+__host__ __device__ constexpr int window_margin(int D) { return D <= 8 ? 8 : -1; }
 __host__ __device__ constexpr int window_out_cols(int D) {
-  return 128 - 2 * (D + (D & 1));
+  return 128 - 2 * window_margin(D);
 }
 
 struct PanelW {
@@ -614,7 +617,7 @@ struct PanelW {
     h = h_;
     cend = cend_;
     ring = rs;
-    const int H = D + (D & 1);
+    const int H = window_margin(D);
     const int lane = __lane_id();
     c = c0 - H + 2 * lane;
     vload = c <= w + D;
@@ -632,13 +635,15 @@ struct PanelW {
   // One sweep: row q of the next grid from three rows of the current one
   // (COPY: the row itself, a stage that does no sweep).  CHECK = false is
   // for rows and waves that meet no ring cell.
-  // RS: add the sweep's exact residual over the cells this window stores
-  // on rows [q0, q1) to acc.
-  template <bool CHECK, bool RS = false>
+  // RS = 1: add the sweep's exact residual over the cells this window
+  // stores on rows [q0, q1) to *acc; RS = 2: add the plain FP64 sum of
+  // (u' - u)^2 over them to *accd (a bound for the stop filter, not a
+  // result).
+  template <bool CHECK, int RS = 0>
   __device__ inline void stage(const RowW& up, const RowW& cu, const RowW& dn,
                                int q, RowW& o, bool copy = false,
-                               u128acc* acc = nullptr, int q0 = 0,
-                               int q1 = 0) const {
+                               u128acc* acc = nullptr, double* accd = nullptr,
+                               int q0 = 0, int q1 = 0) const {
     if (copy ||
         (CHECK && ((q == 0 && ring.top) || (q == h + 1 && ring.bottom)))) {
       o = cu;
@@ -655,26 +660,41 @@ struct PanelW {
       if (ring_b) nb = b;
     }
     o.v = make_double2(na, nb);
-    if (RS && q >= q0 && q < q1 && vstore) {
-      if (c <= cend) acc_add(*acc, res_term(na, a));
-      if (c + 1 <= cend) acc_add(*acc, res_term(nb, b));
+    if (RS != 0 && q >= q0 && q < q1 && vstore) {
+      if (RS == 1) {
+        if (c <= cend) acc_add(*acc, res_term(na, a));
+        if (c + 1 <= cend) acc_add(*acc, res_term(nb, b));
+      } else {
+        add_sq(*accd, na, a, nb, b);
+      }
     }
+  }
+
+  // The FP64 sum of squared changes of the stored cells of a pair.
+  __device__ inline void add_sq(double& accd, double na, double a, double nb,
+                                double b) const {
+    const double da = na - a, db = nb - b;
+    if (c <= cend) accd += da * da;
+    if (c + 1 <= cend) accd += db * db;
   }
 
   // The stored sweep: row `row` of the output, its residual against the
   // input of the sweep, and the outgoing edges D deep.
-  template <int D, bool RES, bool PACK>
+  // (APX: the FP64 sum of squared changes into *accd as well)
+  template <int D, bool RES, bool PACK, bool APX = false>
   __device__ inline void outrow(const RowW& up, const RowW& cu, const RowW& dn,
-                                int row, u128acc& acc,
-                                const EdgeOut2& eo) const {
+                                int row, u128acc& acc, const EdgeOut2& eo,
+                                double* accd = nullptr) const {
     const double a = cu.v.x, b = cu.v.y;
     const double bl = lane_shr1z(b);
     const double ar = lane_shl1z(a);
     const double na = (a + bl + b + up.v.x + dn.v.x) * 0.2;
     const double nb = (b + a + ar + up.v.y + dn.v.y) * 0.2;
-    if (vstore)
+    if (vstore) {
       store_pair<D, RES, PACK>(out + (int64_t)row * pitch + kColOff, c, cend,
                                w, h, row, na, nb, a, b, acc, eo);
+      if (APX) add_sq(*accd, na, a, nb, b);
+    }
   }
 };
 
@@ -687,18 +707,21 @@ struct PanelW {
 // static indices because the step loop is unrolled by three.  RES adds the
 // exact residual of the last sweep.  n_copy (0 .. D-1) leading stages only
 // copy their input, so one instantiation also does D - n_copy sweeps (the
-// halos read stay D deep).  RALL (with RES): stage s = 1 .. D-1 also adds
-// its sweep's residual over the stored cells to accs[s-1] (acc is the last
-// stage's).
-template <int D, bool RES, bool PACK, bool RALL = false>
+// halos read stay D deep).  RALL = 1 (with RES): stage s = 1 .. D-1 also
+// adds its sweep's exact residual over the stored cells to accs[s-1] (acc
+// is the last stage's); RALL = 2: every stage s = 1 .. D adds the FP64 sum
+// of its squared changes over the stored cells to accd[s-1].
+template <int D, bool RES, bool PACK, int RALL = 0>
 __device__ inline void sweep_panelD(const double* __restrict__ in,
                                     double* __restrict__ out, int64_t pitch,
                                     int w, int h, int c0, int r0, int r1,
                                     u128acc& acc, const EdgeOut2& eo,
                                     RingSides rs, int cend = -1,
-                                    int n_copy = 0, u128acc* accs = nullptr) {
-  static_assert(!RALL || RES, "per-sweep residuals include the last one");
-  static_assert(D >= 3 && D <= 14, "deep fused sweep: 3 <= D <= 14");
+                                    int n_copy = 0, u128acc* accs = nullptr,
+                                    double* accd = nullptr) {
+  static_assert(RALL != 1 || RES, "per-sweep residuals include the last one");
+  static_assert(RALL != 2 || !RES, "the FP64 sums replace the exact ones");
+  static_assert(D >= 3 && D <= 8, "deep fused sweep: 3 <= D <= 8");
   if (r0 >= r1) return;
   PanelW p;
   p.setup(in, out, pitch, w, h, c0, cend < 0 ? w : cend, rs, D);
@@ -729,7 +752,7 @@ __device__ inline void sweep_panelD(const double* __restrict__ in,
       if (t < n_steps) {
         const int q1 = r0 - D + 1 + t;
         p.template stage<true, RALL>(R[k], R[(k + 1) % 3], R[(k + 2) % 3], q1,
-                                     S[0][k], n_copy >= 1, accs, r0, r1);
+                                     S[0][k], n_copy >= 1, accs, accd, r0, r1);
         if (q1 + 2 <= r1 + D - 1) p.load(R[k], q1 + 2);
 #pragma unroll
         for (int s = 2; s < D; s++)
@@ -737,12 +760,13 @@ __device__ inline void sweep_panelD(const double* __restrict__ in,
             p.template stage<true, RALL>(S[s - 2][(k + 1) % 3],
                                          S[s - 2][(k + 2) % 3], S[s - 2][k],
                                          q1 - (s - 1), S[s - 1][k], n_copy >= s,
-                                         RALL ? accs + (s - 1) : nullptr, r0,
-                                         r1);
+                                         RALL == 1 ? accs + (s - 1) : nullptr,
+                                         RALL == 2 ? accd + (s - 1) : nullptr,
+                                         r0, r1);
         if (t >= 2 * (D - 1))
-          p.template outrow<D, RES, PACK>(S[D - 2][(k + 1) % 3],
-                                          S[D - 2][(k + 2) % 3], S[D - 2][k],
-                                          q1 - (D - 1), acc, eo);
+          p.template outrow<D, RES, PACK, RALL == 2>(
+              S[D - 2][(k + 1) % 3], S[D - 2][(k + 2) % 3], S[D - 2][k],
+              q1 - (D - 1), acc, eo, RALL == 2 ? accd + (D - 1) : nullptr);
       }
     }
   };
@@ -757,17 +781,19 @@ __device__ inline void sweep_panelD(const double* __restrict__ in,
       for (int k = 0; k < 3; k++) {
         const int q1 = r0 - D + 1 + t0 + k;
         p.template stage<false, RALL>(R[k], R[(k + 1) % 3], R[(k + 2) % 3],
-                                      q1, S[0][k], n_copy >= 1, accs, r0, r1);
+                                      q1, S[0][k], n_copy >= 1, accs, accd, r0,
+                                      r1);
         p.load(R[k], q1 + 2);
 #pragma unroll
         for (int s = 2; s < D; s++)
           p.template stage<false, RALL>(
               S[s - 2][(k + 1) % 3], S[s - 2][(k + 2) % 3], S[s - 2][k],
               q1 - (s - 1), S[s - 1][k], n_copy >= s,
-              RALL ? accs + (s - 1) : nullptr, r0, r1);
-        p.template outrow<D, RES, PACK>(S[D - 2][(k + 1) % 3],
-                                        S[D - 2][(k + 2) % 3], S[D - 2][k],
-                                        q1 - (D - 1), acc, eo);
+              RALL == 1 ? accs + (s - 1) : nullptr,
+              RALL == 2 ? accd + (s - 1) : nullptr, r0, r1);
+        p.template outrow<D, RES, PACK, RALL == 2>(
+            S[D - 2][(k + 1) % 3], S[D - 2][(k + 2) % 3], S[D - 2][k],
+            q1 - (D - 1), acc, eo, RALL == 2 ? accd + (D - 1) : nullptr);
       }
     }
   }

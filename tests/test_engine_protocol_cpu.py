@@ -4,16 +4,20 @@ forms), run under random interleavings and checked against the oracle.
 The model follows the device code of charming_amd/csrc/jacobi2d/batch.hip
 step for step - per-tile counters for the next even/odd round, re-arming at
 task start, up to eight neighbours, rounds of `depth` fused sweeps (eight
-for a fixed sweep count, four with a tolerance, two for small tiles), the
-stop-decision gate, the per-sweep residuals, the repair of a stop inside a
-fused round, the in-flight count and the state machine - while the stencil
-work is done with numpy on the shared arrays.  Every task is split into a read phase and a write phase with other
+for a fixed sweep count, six with a tolerance, two for small tiles), the
+stop-decision gate, the per-sweep residuals (exact in the first and last
+round of a tolerance problem, an FP64 filter in between, with an exact
+recompute of a round the filter cannot rule out while its last tile's
+release waits), the repair of a stop inside a fused round, the in-flight
+count and the state machine - while the stencil work is done with numpy on
+the shared arrays.  Every task is split into a read phase and a write phase with other
 tasks free to run in between, so two tasks the protocol lets overlap do
 overlap.  If the protocol ever let a tile read cells a neighbour had not
 finished, or overwrite cells a neighbour still needed, the result would
 differ from the monolithic oracle's.
 """
 import random
+import sys
 
 import numpy as np
 import pytest
@@ -52,9 +56,9 @@ def sweep_region(src, r0, r1, c0, c1, H, W):
 
 
 def depth_for(W, H, bs, tol2):
-    """Sweeps per round (build_request): 8 for a fixed sweep count, 4 with a
+    """Sweeps per round (build_request): 8 for a fixed sweep count, 6 with a
     tolerance, 2 when the tile or grid is thinner than that."""
-    deep = 8 if tol2 < 0 else 4
+    deep = 8 if tol2 < 0 else 6
     return deep if min(W, H, bs) >= deep else 2
 
 
@@ -78,6 +82,10 @@ class Problem:
         self.fix_sweeps = 0
         self.round_tiles = [0, 0]
         self.res = [[0] * 8, [0] * 8]
+        self.resd = [[0.0] * 8, [0.0] * 8]
+        self.resid_round = None
+        self.resid_tiles = 0
+        self.held_tile = None
         self.s0 = 0
         self.stop_iters = 0
         self.s_last = 0
@@ -98,6 +106,28 @@ class Problem:
         ti, tj = divmod(u, self.tc)
         r0, c0 = 1 + ti * self.bs, 1 + tj * self.bs
         return r0, min(r0 + self.bs, self.H + 1), c0, min(c0 + self.bs, self.W + 1)
+
+
+def filtered_round(p, k):
+    return p.tol2 >= 0 and p.depth >= 3 and 0 < k < p.rounds - 1
+
+
+def filter_may_stop(p, sk, n_sw):
+    """The device's bound: A 2^56 (1 - 2^-20) - N > thr (1 + 2^-20) proves
+    that the exact residual exceeds thr."""
+    thr = p.tol2 * u128d(p.s0)
+    if not thr < 2.0 ** 62:
+        return True
+    n = float(p.W * p.H)
+    for j in range(n_sw):
+        a = p.resd[sk][j]
+        if not (a * 2.0 ** 56 * (1 - 2.0 ** -20) - n > thr * (1 + 2.0 ** -20)):
+            return True
+    return False
+
+
+# force the exact recompute of every filtered round (set by a test)
+ALWAYS_RECOMPUTE = False
 
 
 class Engine:
@@ -122,6 +152,8 @@ class Engine:
             gate = 1 if p.tol2 >= 0 else 0  # round k+2 waits for decision k
             p.pend[u][k & 1] = 1 + len(p.nbrs(u)) + gate
             return k
+        if kind == "resid":
+            return p.resid_round
         return p.fix_round if kind == "fix" else None
 
     @staticmethod
@@ -141,6 +173,21 @@ class Engine:
             big[a0:a1, b0:b1] = new
         return big[r0:r1, c0:c1].copy(), res
 
+    @staticmethod
+    def fused_sq(p, src, n_sw, r0, r1, c0, c1):
+        """Like fused, the FP64 sums of squared changes of every sweep."""
+        big = src.copy()
+        out = []
+        for s in range(1, n_sw + 1):
+            e = n_sw - s
+            a0, a1 = max(r0 - e, 0), min(r1 + e, p.H + 2)
+            b0, b1 = max(c0 - e, 0), min(c1 + e, p.W + 2)
+            new = sweep_region(big, a0, a1, b0, b1, p.H, p.W)
+            d = new[r0 - a0:r1 - a0, c0 - b0:c1 - b0] - big[r0:r1, c0:c1]
+            out.append(float(np.sum(d * d)))
+            big[a0:a1, b0:b1] = new
+        return out
+
     # ---- the read phase: at the task's start or just before its end ----
     def read(self, t, k):
         kind, pi, u = t
@@ -150,10 +197,14 @@ class Engine:
             n_sw = min(p.depth, p.iters - p.depth * k)
             tol, last = p.tol2 >= 0, k + 1 == p.rounds
             new, res = self.fused(p, p.A[k % 3], n_sw, r0, r1, c0, c1)
-            # what the device adds: every sweep's residual with a tolerance;
-            # on the last round of a fixed count the last one, into the
-            # second slot for a pass of two or more sweeps
-            if tol:
+            # what the device adds: every sweep's residual with a tolerance
+            # (FP64 sums of squared changes in filtered rounds); on the last
+            # round of a fixed count the last one, into the second slot for
+            # a pass of two or more sweeps
+            if tol and filtered_round(p, k):
+                add = ("approx", self.fused_sq(p, p.A[k % 3], n_sw, r0, r1,
+                                               c0, c1))
+            elif tol:
                 add = res
             elif last:
                 add = [res[-1]] if n_sw == 1 else [0, res[-1]]
@@ -163,6 +214,10 @@ class Engine:
         if kind == "fix":
             new, _ = self.fused(p, p.A[k % 3], p.fix_sweeps, r0, r1, c0, c1)
             return (k, new, [])
+        if kind == "resid":
+            n_sw = min(p.depth, p.iters - p.depth * k)
+            new, res = self.fused(p, p.A[k % 3], n_sw, r0, r1, c0, c1)
+            return (k, new, res)
         res_arr = ((p.stop_iters + p.depth - 1) // p.depth) % 3
         return (None, O.grid_hash(p.A[res_arr], r0, r1, c0, c1), [])
 
@@ -176,20 +231,39 @@ class Engine:
             p.A[(k + 1) % 3][r0:r1, c0:c1] = new
             p.pend[u][2] = k + 1
             flags = self.complete_round(p, pi, u, k, add)
-            if flags == "gate" and k + 2 < p.rounds:
-                for v in range(p.n):
-                    p.pend[v][k & 1] -= 1
-                    if p.pend[v][k & 1] == 0:
-                        p.inflight += 1
-                        self.push(("iter", pi, v))
+            if flags == "gate":
+                self.gate(p, pi, k)
         elif kind == "fix":
             p.A[(k + 1) % 3][r0:r1, c0:c1] = new
+        elif kind == "resid":
+            # the recompute writes what the round wrote
+            assert np.array_equal(p.A[(k + 1) % 3][r0:r1, c0:c1], new)
+            p.A[(k + 1) % 3][r0:r1, c0:c1] = new
+            sk = k & 1
+            for j, v in enumerate(add):
+                p.res[sk][j] += v
+            p.resid_tiles += 1
+            if p.resid_tiles == p.n:
+                flags = self.decide_exact(p, k, sk)
+                if p.state == 0:
+                    self.release(p, pi, p.held_tile, k)
+                    if flags == "gate":
+                        self.gate(p, pi, k)
         else:
             p.hash = (p.hash + new) % 2 ** 64
         self.finish_task(p, pi)
 
-    def complete_round(self, p, pi, u, k, add):
-        flags = self.decide(p, k, add)
+    def gate(self, p, pi, k):
+        """The decision on round k lets round k+2 of every tile go."""
+        if k + 2 < p.rounds:
+            for v in range(p.n):
+                p.pend[v][k & 1] -= 1
+                if p.pend[v][k & 1] == 0:
+                    p.inflight += 1
+                    self.push(("iter", pi, v))
+
+    def release(self, p, pi, u, k):
+        """Tile u's round k is done: its and its neighbours' round k+1."""
         if k + 1 < p.rounds:
             for v in [u] + p.nbrs(u):
                 p.pend[v][(k + 1) & 1] -= 1
@@ -197,6 +271,16 @@ class Engine:
                 if p.pend[v][(k + 1) & 1] == 0:
                     p.inflight += 1
                     self.push(("iter", pi, v))
+
+    def complete_round(self, p, pi, u, k, add):
+        flags = self.decide(p, k, add)
+        if flags == "resid":
+            # every tile recomputes round k exactly; u's release waits
+            p.held_tile = u
+            for v in range(p.n):
+                self.push(("resid", pi, v))
+            return None
+        self.release(p, pi, u, k)
         return flags
 
     def decide(self, p, k, add):
@@ -204,13 +288,29 @@ class Engine:
         if not tol and not last:
             return None
         sk = (k & 1) if tol else 0
-        for j, v in enumerate(add):
-            p.res[sk][j] += v
+        if isinstance(add, tuple):
+            for j, v in enumerate(add[1]):
+                p.resd[sk][j] += v
+        else:
+            for j, v in enumerate(add):
+                p.res[sk][j] += v
         p.round_tiles[sk] += 1
         if p.round_tiles[sk] != p.n:
             return None
         if p.state != 0:
             return None  # a stop was decided on the previous round
+        if filtered_round(p, k):
+            if ALWAYS_RECOMPUTE or filter_may_stop(p, sk, p.depth):
+                p.resid_round, p.resid_tiles, p.state = k, 0, 5
+                p.inflight += p.n
+                return "resid"
+            p.round_tiles[sk] = 0
+            p.resd[sk] = [0.0] * 8
+            return "gate"
+        return self.decide_exact(p, k, sk)
+
+    def decide_exact(self, p, k, sk):
+        tol, last = p.tol2 >= 0, k + 1 == p.rounds
         n_sw = min(p.depth, p.iters - p.depth * k)
         stop = 0
         if not tol:
@@ -230,6 +330,8 @@ class Engine:
             return None
         p.round_tiles[sk] = 0
         p.res[sk] = [0] * 8
+        p.resd[sk] = [0.0] * 8
+        p.state = 0
         return "gate"
 
     def finish_task(self, p, pi):
@@ -269,7 +371,8 @@ CASES = [  # W, H, tile, sweeps, tol
     (50, 40, 16, 400, 3e-2), (24, 24, 8, 400, 0.5), (30, 18, 8, 7, 1e-9),
     (17, 33, 16, 400, 1e-2), (12, 12, 4, 2, 0.0), (40, 40, 16, 29, 0.0),
     (26, 19, 8, 400, 2e-2), (45, 21, 8, 400, 7e-3), (20, 20, 4, 15, 0.0),
-    (24, 24, 8, 400, 5e-2), (30, 22, 8, 400, 0.3)]
+    (24, 24, 8, 400, 5e-2), (30, 22, 8, 400, 0.3),
+    (12, 12, 4, 160, 1e-13)]   # converges below the filter's bound
 
 
 @pytest.mark.parametrize("seed", [1, 2, 3, 4, 5, 6])
@@ -287,6 +390,21 @@ def test_protocol_matches_oracle_under_random_interleavings(seed):
         assert p.stop_iters == n, (W, H, bs, it, tol)
         assert p.hash == O.grid_hash(g), (W, H, bs, it, tol)
         assert p.s_last == s_last, (W, H, bs, it, tol)
+
+
+def test_protocol_with_every_filtered_round_recomputed(monkeypatch):
+    # the exact recompute, the held release and the deferred gate on every
+    # filtered round (not only where a stop may lie) still give the oracle's
+    # answers
+    monkeypatch.setattr(sys.modules[__name__], "ALWAYS_RECOMPUTE", True)
+    rng = random.Random(11)
+    cases = [c for c in CASES if c[4] > 0]
+    probs = [Problem(W, H, bs, it, tol, 500 + i)
+             for i, (W, H, bs, it, tol) in enumerate(cases)]
+    Engine(probs, rng).run(max_overlap=12)
+    for i, ((W, H, bs, it, tol), p) in enumerate(zip(cases, probs)):
+        g, n, s_last, s0 = O.jacobi_solve_tol(W, H, it, tol, seed=500 + i)
+        assert (p.stop_iters, p.hash, p.s_last) == (n, O.grid_hash(g), s_last)
 
 
 def test_model_detects_a_missing_corner_dependency(monkeypatch):

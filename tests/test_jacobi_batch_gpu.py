@@ -113,15 +113,21 @@ def test_tolerance_stop_matches_oracle(cam):
 
 
 def test_tolerance_stop_at_every_sweep_of_a_round(cam):
-    # tolerance rounds fuse four sweeps and sum each one's residual; a stop
-    # on sweep 1, 2 or 3 of a round is repaired by redoing that many sweeps
-    # from the round's input, a stop on sweep 4 keeps the round's output;
-    # caps that leave a short last round (6, 7, 9 sweeps) too
+    # tolerance rounds fuse six sweeps (an FP64 filter in the rounds between
+    # the first and the last, exact residuals where it cannot rule a stop
+    # out); a stop on sweep 1..5 of a round is repaired by redoing that many
+    # sweeps from the round's input, a stop on sweep 6 keeps the round's
+    # output; caps that leave a short last round (7, 9, 10 sweeps) too
     tols = [0.5 * 0.7 ** k for k in range(16)]
     probs = [{"width": 48, "height": 40, "iters": 400, "tol": t, "seed": 31,
               "block": 16} for t in tols]
     probs += [{"width": 40, "height": 44, "iters": it, "tol": 1e-12, "seed": it,
-               "block": 16} for it in (6, 7, 9)]
+               "block": 16} for it in (7, 9, 10)]
+    # small grids that converge below the FP64 filter's bound: the rounds
+    # it cannot rule out are recomputed exactly, mostly without a stop,
+    # until the exact residual reaches 0 (8x8, 10x8) or the cap (8x12)
+    probs += [{"width": w, "height": h, "iters": 400, "tol": 1e-13,
+               "seed": w * h, "block": 16} for w, h in ((8, 8), (8, 12), (10, 8))]
     out = cam.batch_run(probs)
     stops = set()
     for i, p in enumerate(probs):
@@ -130,8 +136,9 @@ def test_tolerance_stop_at_every_sweep_of_a_round(cam):
         assert out["iters"][i] == n, (i, p)
         assert out["hashes"][i] == O.grid_hash(g), (i, p)
         assert out["residuals"][i] == s_last, (i, p)
-        stops.add(n % 4)
-    assert stops == {0, 1, 2, 3}
+        stops.add(n % 6)
+    assert stops == {0, 1, 2, 3, 4, 5}
+    assert out["iters"][-3:] == [328, 400, 387]
 
 
 def test_plain_and_graph_forms_match_engine(cam):

@@ -29,8 +29,12 @@ def shl1z(v):
     return o
 
 
+def window_margin(D):
+    return 8   # D <= 8: 112 output columns, seven whole 128-byte lines
+
+
 def window_out_cols(D):
-    return 128 - 2 * (D + (D & 1))
+    return 128 - 2 * window_margin(D)
 
 
 class Chare:
@@ -47,7 +51,7 @@ class Chare:
 
 def sweep_window(D, inp, out, w, h, c0, r0, r1, rs, acc, eo, pack):
     """sweep_panelD<D, RES, PACK> on one band of one window."""
-    H = D + (D & 1)
+    H = window_margin(D)
     lane = np.arange(64)
     c = c0 - H + 2 * lane
     vload = c <= w + D
