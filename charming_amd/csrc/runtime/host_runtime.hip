@@ -36,8 +36,18 @@ struct Runtime {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   uint32_t* host_abort = nullptr;  // host-coherent
   std::vector<void*> dev_allocs;
+  Stats* stats_buf = nullptr;  // n_pes packed statistics (chm_get_stats)
   chm_stats last;
 };
+
+// Packs the statistics of every PE (one field of each PeState) into one
+// contiguous buffer, so reading them is a single small copy rather than a
+// strided copy with one row per PE (milliseconds at 256 PEs).
+__global__ void gather_stats_kernel(const PeState* pes, Stats* out, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x)
+    out[i] = pes[i].stats;
+}
 
 Runtime& R() {
   static Runtime r;
@@ -97,6 +107,7 @@ void free_all() {
   if (r.stream) (void)hipStreamDestroy(r.stream);
   r.ev0 = r.ev1 = nullptr;
   r.stream = nullptr;
+  r.stats_buf = nullptr;
   r.inited = false;
 }
 
@@ -232,6 +243,7 @@ int chm_init(int argc, char** argv, const chm_config* cfg_in) {
   CHM_STEP(dev_alloc(&h.heap, (size_t)n_pes * cfg.heap_bytes, false));
   CHM_STEP(dev_alloc(&h.ctl, charm::rt::CTL_WORDS * sizeof(uint32_t), true));
   CHM_STEP(dev_alloc(&h.heartbeat, (size_t)n_pes * 128, true));
+  CHM_STEP(dev_alloc(&r.stats_buf, (size_t)n_pes * sizeof(Stats), true));
   // argv in device memory (charming.cu:145-169 used managed memory)
   size_t total = 0;
   for (int i = 0; i < argc; i++) total += strlen(argv[i]) + 1;
@@ -383,9 +395,12 @@ int chm_get_stats(chm_stats* out) {
   if (!out) return chm::fail(CHM_E_INVALID, "null stats pointer");
   if (!r.inited) return chm::fail(CHM_E_STATE, "chm_init first");
   std::vector<Stats> s(r.h.n_pes);
-  CHM_TRY_HIP(hipMemcpy2D(s.data(), sizeof(Stats), &r.h.pes[0].stats,
-                          sizeof(PeState), sizeof(Stats), r.h.n_pes,
-                          hipMemcpyDeviceToHost));
+  gather_stats_kernel<<<(r.h.n_pes + 255) / 256, 256, 0, r.stream>>>(
+      r.h.pes, r.stats_buf, r.h.n_pes);
+  CHM_TRY_HIP(hipGetLastError());
+  CHM_TRY_HIP(hipMemcpyAsync(s.data(), r.stats_buf, sizeof(Stats) * s.size(),
+                             hipMemcpyDeviceToHost, r.stream));
+  CHM_TRY_HIP(hipStreamSynchronize(r.stream));
   chm_stats o = r.last;
   o.delivered = o.sent = o.deferred = o.revived = o.send_spins = 0;
   double ticks = 0;
