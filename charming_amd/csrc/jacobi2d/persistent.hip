@@ -123,6 +123,7 @@ struct Block : charm::chare {
   __device__ size_t halo_bytes(int k) const;
   __device__ void fill_corners(double* const* bufs);
   __device__ void send_boundaries();
+  __device__ void put_edges(const double* g, int D, int parity);
   __device__ void recv_ghosts(void* arg);
   __device__ void update();
   __device__ void end(void* arg);
@@ -559,10 +560,73 @@ __device__ void Block::send_boundaries() {
   }
 }
 
+// Deep rounds with zero-copy halos: the workgroup writes the new grid's
+// edges, D deep, and its (D-1) x (D-1) corner blocks straight into the halo
+// of each neighbour's array of the same parity (the one its next round
+// reads), exactly where the neighbour's unpack would put them.  No
+// conflict: the neighbour's current round reads its other array and writes
+// only the interior of this one, and its previous round (which read this
+// halo) has ended, or this chare could not have run its own.  The messages
+// that follow only notify (slot word 1 = 1: placed).  g was just written by
+// other waves, partly with streaming stores: loads at agent scope.
+__device__ void Block::put_edges(const double* g, int D, int parity) {
+  const JacobiDesc& d = g_jdesc;
+  const int bw = block_width, bh = block_height;
+  const int64_t P = pitch;
+  for (int k = 0; k < N_NEIGHBORS; k++) {
+    const int n = neighbor_index[k];
+    if (n < 0) continue;
+    const ChareGeom gn = chare_geom(d.W, d.H, d.cr, d.cc, n);
+    const int64_t PN = chare_pitch(gn.bw);
+    double* t = d.arena + (int64_t)(2 * n + parity) * d.chare_elems +
+                (kHaloRows - 1) * PN;
+    // rows x columns of the source block in g (from si0, sj0) and of its
+    // place in the neighbour's halo (from di0, dj0)
+    int si0, sj0, di0, dj0, ni, nj;
+    switch (k) {
+      case LEFT:  // its right halo
+        si0 = 1; sj0 = 1; di0 = 1; dj0 = gn.bw + 1; ni = bh; nj = D; break;
+      case RIGHT:  // its left halo
+        si0 = 1; sj0 = bw - D + 1; di0 = 1; dj0 = 1 - D; ni = bh; nj = D;
+        break;
+      case TOP:  // its bottom halo
+        si0 = 1; sj0 = 1; di0 = gn.bh + 1; dj0 = 1; ni = D; nj = bw; break;
+      case BOTTOM:  // its top halo
+        si0 = bh - D + 1; sj0 = 1; di0 = 1 - D; dj0 = 1; ni = D; nj = bw;
+        break;
+      case TOP_LEFT:  // its bottom-right corner
+        si0 = 1; sj0 = 1; di0 = gn.bh + 1; dj0 = gn.bw + 1; ni = nj = D - 1;
+        break;
+      case TOP_RIGHT:  // its bottom-left corner
+        si0 = 1; sj0 = bw - D + 2; di0 = gn.bh + 1; dj0 = 2 - D;
+        ni = nj = D - 1;
+        break;
+      case BOTTOM_LEFT:  // its top-right corner
+        si0 = bh - D + 2; sj0 = 1; di0 = 2 - D; dj0 = gn.bw + 1;
+        ni = nj = D - 1;
+        break;
+      default:  // BOTTOM_RIGHT: its top-left corner
+        si0 = bh - D + 2; sj0 = bw - D + 2; di0 = 2 - D; dj0 = 2 - D;
+        ni = nj = D - 1;
+        break;
+    }
+    for (int e = threadIdx.x; e < ni * nj; e += blockDim.x) {
+      const int i = e / nj, j = e % nj;
+      const uint64_t v = __hip_atomic_load(
+          (const uint64_t*)(g + (int64_t)(si0 + i) * P + kColOff + sj0 + j),
+          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      t[(int64_t)(di0 + i) * PN + kColOff + dj0 + j] =
+          __longlong_as_double((long long)v);
+    }
+  }
+}
+
 __device__ void Block::recv_ghosts(void* arg) {
-  // (jacobi2d.cu:196-229) unpack into the halo of u_iter
+  // (jacobi2d.cu:196-229) unpack into the halo of u_iter (unless the
+  // sender placed the cells itself: slot word 1)
   const uint64_t tr0 = wall_clock64();
   const int dir = *(const int*)arg;
+  const bool placed = ((const int*)arg)[1] == 1;
   const double* ghost = (const double*)arg + 1;
   double* t = grids[cur];
   const int bw = block_width, bh = block_height;
@@ -570,7 +634,9 @@ __device__ void Block::recv_ghosts(void* arg) {
   const int D = depth;
   // edge messages hold `depth` rows/columns, nearest the sender's far side
   // first: they land in halo rows/columns 1-D .. 0 or n+1 .. n+D
-  switch (dir) {
+  if (dir < 0 || dir >= N_NEIGHBORS) {
+    if (threadIdx.x == 0) charm::rt::raise_error(charm::rt::ERR_BAD_MESSAGE, dir);
+  } else if (!placed) switch (dir) {
     case LEFT:
       for (int k = threadIdx.x; k < D * bh; k += blockDim.x)
         t[P * (1 + k % bh) + kColOff + 1 - D + k / bh] = ghost[k];
@@ -638,6 +704,9 @@ __device__ void Block::update() {
     // neighbours' inboxes first and the sweep writes the edges straight
     // into them (no staging copy); sent after the sweep
     const bool zc = !ov && !last && g_jdesc.halo_copy == 0;
+    // deep rounds put their edges straight into the neighbours' halos
+    // (put_edges): their messages only notify
+    const bool put = zc && steps >= 3;
     // (kept in LDS: the values are the workgroup's, and the stack of an
     // entry method is small)
     __shared__ charm::envelope* s_env[N_NEIGHBORS];
@@ -647,14 +716,15 @@ __device__ void Block::update() {
       double* d = (neighbor_index[k] >= 0) ? boundaries[k] : nullptr;
       if (zc && neighbor_index[k] >= 0) {
         charm::message m;
-        d = (double*)block_proxy->alloc(m, neighbor_index[k], 1, halo_bytes(k),
+        d = (double*)block_proxy->alloc(m, neighbor_index[k], 1,
+                                        put ? sizeof(double) : halo_bytes(k),
                                         iter + steps);
         if (threadIdx.x == 0) {
           s_env[k] = m.env;
           s_off[k] = m.offset;
           if (d) {
             ((int*)d)[0] = opposite_side(k);
-            ((int*)d)[1] = 0;
+            ((int*)d)[1] = put ? 1 : 0;
           }
         }
       }
@@ -675,7 +745,10 @@ __device__ void Block::update() {
         sweepD_dispatch(steps, t, nt, P, bw, bh, last, rs);
         if (!last) {
           __syncthreads();
-          pack_edges(nt, P, bw, bh, steps, eo2);
+          if (put)
+            put_edges(nt, steps, cur ^ 1);
+          else
+            pack_edges(nt, P, bw, bh, steps, eo2);
         }
       } else
         sweep2_dispatch(t, nt, P, bw, bh, last, !last, eo2, rs);
@@ -698,7 +771,7 @@ __device__ void Block::update() {
     __syncthreads();
     // the halos of u_{iter} leave before the interior is swept
     if (!done && zc) {
-      if (threadIdx.x == 0) fill_corners(s_dst);
+      if (threadIdx.x == 0 && !put) fill_corners(s_dst);
       for (int k = 0; k < N_NEIGHBORS; k++) {
         if (neighbor_index[k] < 0) continue;
         charm::message m;
