@@ -50,6 +50,11 @@ EXES = [
     ("async_wait", ["examples/async_wait.hip"]),
 ]
 MAIN_SRC = "runtime/main.cpp"
+# standalone programs (their own main, no runtime): the raw mailbox pingpong
+# that gives the runtime's message latency its floor
+STANDALONE_EXES = [
+    ("raw_pingpong", ["examples/raw_pingpong.hip"]),
+]
 BINDIR = os.path.join(PKG, "bin")
 
 
@@ -86,6 +91,16 @@ def _link_exe(name, srcs, verbose):
     cmd = [HIPCC, "--offload-arch=gfx950", "-o", os.path.join(BINDIR, name)]
     cmd += [_obj(s) for s in srcs] + [_obj(MAIN_SRC)]
     cmd += ["-L", LIBDIR, "-lcharming_amd", "-Wl,-rpath,$ORIGIN/../lib", "-lpthread"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError(f"link failed for {name}:\n{p.stdout}{p.stderr}")
+
+
+def _link_standalone(name, srcs, verbose):
+    cmd = [HIPCC, "--offload-arch=gfx950", "-o", os.path.join(BINDIR, name)]
+    cmd += [_obj(s) for s in srcs]
     if verbose:
         print(" ".join(cmd), flush=True)
     p = subprocess.run(cmd, capture_output=True, text=True)
@@ -138,7 +153,8 @@ def build(verbose=False, jobs=None):
     os.makedirs(OBJDIR, exist_ok=True)
     os.makedirs(BINDIR, exist_ok=True)
     srcs = sorted({s for _, ss, _ in LIBS for s in ss} |
-                  {s for _, ss in EXES for s in ss} | {MAIN_SRC})
+                  {s for _, ss in EXES for s in ss} |
+                  {s for _, ss in STANDALONE_EXES for s in ss} | {MAIN_SRC})
     missing = [s for s in srcs if not os.path.exists(os.path.join(CSRC, s))]
     if missing:
         raise FileNotFoundError(f"missing sources: {missing}")
@@ -150,6 +166,8 @@ def build(verbose=False, jobs=None):
         _link(name, ss, deps, verbose)
     for name, ss in EXES:
         _link_exe(name, ss, verbose)
+    for name, ss in STANDALONE_EXES:
+        _link_standalone(name, ss, verbose)
     return LIBDIR
 
 

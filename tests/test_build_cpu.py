@@ -36,3 +36,8 @@ def test_jacobi_abi_version_without_gpu():
 def test_example_programs_built(prog):
     assert os.path.exists(os.path.join(ROOT, "charming_amd", "lib", f"libchm_{prog}.so"))
     assert os.path.exists(os.path.join(ROOT, "charming_amd", "bin", prog))
+
+
+def test_raw_pingpong_built():
+    assert os.access(os.path.join(ROOT, "charming_amd", "bin", "raw_pingpong"),
+                     os.X_OK)

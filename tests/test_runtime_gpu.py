@@ -7,6 +7,7 @@ second process while this one holds the device's cooperative resources."""
 import ctypes
 import os
 import re
+import subprocess
 
 import pytest
 
@@ -80,6 +81,23 @@ def test_pingpong_exact_payloads(capfd):
     assert m and int(m.group(1)) == 18 * 22 and int(m.group(2)) == 0
     assert "peer done: payload errors 0" in out
     assert int(st["deferred"]) == 0 and int(st["error"]) == 0
+
+
+def test_raw_mailbox_pingpong_floor():
+    # the latency floor without the runtime (examples/baseline/baseline.cu):
+    # two workgroups pass flags through per-PE mailboxes; a standalone
+    # program with an ordinary launch, run as a child process
+    exe = os.path.join(ROOT, "charming_amd", "bin", "raw_pingpong")
+    p = subprocess.run([exe, "200", "5"], capture_output=True, text=True,
+                       timeout=60)
+    assert p.returncode == 0, p.stdout + p.stderr
+    got = re.findall(r"PEs 0 and (\d+) of (\d+), 200 round trips: ([\d.]+) us "
+                     r"per one-way trip, errors (\d+)", p.stdout)
+    assert len(got) == 2, p.stdout
+    n = int(got[0][1])
+    assert [int(g[0]) for g in got] == [1, n // 2]
+    for _, _, us, err in got:
+        assert int(err) == 0 and 0.0 < float(us) < 100.0
 
 
 @pytest.mark.parametrize("n_pes", [None, 3])
