@@ -573,7 +573,7 @@ struct RowW {
   double2 v;
 };
 
-// Lane l gets lane l-1's v (l+1's for lane_shl1); the end lane gets 0.
+// Lane l gets lane l-1's v (l+1's for lane_shl1z); the end lane gets 0.
 __device__ inline double lane_shr1z(double v) {
   const long long x = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_mov_dpp((int)x, 0x138, 0xf, 0xf, true);
