@@ -1456,7 +1456,7 @@ struct Server {
 // of them, and tiles grow (fewer fences per cell) as the load rises.
 static int serve_block(int W, int H, int n_wg, int resident) {
   const int64_t target = std::max<int64_t>(1, n_wg / (1 + resident));
-  static const int kSides[] = {512, 256, 128};
+  static const int kSides[] = {448, 224, 112};
   for (int bs : kSides) {
     const int64_t tiles = (int64_t)((W + bs - 1) / bs) * ((H + bs - 1) / bs);
     if (tiles >= target) return bs;
@@ -1645,7 +1645,7 @@ int chm_jacobi2d_batch_autotune(const chm_batch_problem* probs, int n,
                                 int* blocks_out, double* best_rate) {
   if (!probs || !best || n < 1) return fail(CHM_E_INVALID, "null argument");
   const int cap = sample_iters > 0 ? sample_iters : 20;
-  static const int kBlocks[] = {64, 128, 256, 512};
+  static const int kBlocks[] = {64, 112, 128, 224, 256, 448, 512};
   // size classes: one per octave of the longer side
   std::vector<int> cls(n), blocks(n);
   int n_cls = 0;
