@@ -189,7 +189,7 @@ int chm_jacobi2d_batch_plain(const chm_batch_problem* probs, int n,
                              int* iters_done);
 
 /* Autotuner: picks the workgroup size and sweep shape, and a tile side per
- * problem from {64, 128, 256, 512} (blocks_out[i]), by coordinate descent
+ * problem from {64, 112, 128, 224, 256, 448, 512} (blocks_out[i]), by coordinate descent
  * over size classes, timing short engine runs of the whole batch capped at
  * sample_iters sweeps.  best_rate = cell updates per second of the best. */
 int chm_jacobi2d_batch_autotune(const chm_batch_problem* probs, int n,

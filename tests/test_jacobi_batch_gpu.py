@@ -76,7 +76,8 @@ def test_problem_alone_equals_in_mix(cam):
 
 def test_tiles_do_not_change_results(cam):
     base = {"width": 700, "height": 450, "iters": 13, "seed": 77}
-    outs = [cam.batch_run([dict(base, block=b)]) for b in (16, 64, 128, 256, 512, 1024)]
+    outs = [cam.batch_run([dict(base, block=b)])
+            for b in (16, 64, 112, 128, 224, 256, 448, 512, 1024)]
     h, res = oracle_fixed(base)
     for o in outs:
         assert o["hashes"][0] == h and o["residuals"][0] == res
@@ -181,7 +182,7 @@ def test_autotune_small_batch(cam):
     cfg, blocks, rate = cam.batch_autotune(probs, sample_iters=5)
     assert cfg["wg_threads"] in (256, 512, 1024)
     assert (cfg["panel_chunks"], cfg["rows_in_flight"]) in ((1, 2), (1, 4), (2, 2))
-    assert all(b in (64, 128, 256, 512) for b in blocks)
+    assert all(b in (64, 112, 128, 224, 256, 448, 512) for b in blocks)
     assert rate > 0
     tuned = [dict(p, block=b) for p, b in zip(probs, blocks)]
     out = cam.batch_run(tuned, wg_threads=cfg["wg_threads"],
