@@ -897,14 +897,15 @@ static double now_s() {
 }
 
 // Tile side for a problem: `block` (> 0), or a rule that keeps tiles large
-// enough to amortise a task's fences and small enough to spread the problem.
+// enough to amortise a task's fences and small enough to spread the problem,
+// in multiples of the deep pass's 112-column window (measured on the
+// 512-problem batch: 116-118 ms, against 158 ms with sides 64/128/256/512).
 static int pick_block(int W, int H, int block) {
   if (block > 0) return block;
   const int side = std::max(W, H);
-  if (side <= 256) return 64;
-  if (side <= 1024) return 128;
-  if (side <= 2048) return 256;
-  return 512;
+  if (side <= 256) return 112;
+  if (side <= 1024) return 224;
+  return 448;
 }
 
 struct Built {
