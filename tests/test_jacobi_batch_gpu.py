@@ -163,8 +163,9 @@ def test_batch_workload_sample_matches_oracle(cam):
 
 
 def test_large_problem_in_engine_matches_persistent_and_plain(cam):
-    # 4096^2 with 512^2 tiles next to small problems; compare with the plain
-    # form's device hash (the oracle would take minutes here)
+    # 4096^2 with the default 448^2 tiles (the last row and column of tiles
+    # 64 wide) next to small problems; compare with the plain form's device
+    # hash (the oracle would take minutes here)
     big = {"width": 4096, "height": 4096, "iters": 30, "seed": 11}
     small = mixed_problems(20, 5, side_hi=200, it_hi=30)
     out = cam.batch_run([big] + small)
