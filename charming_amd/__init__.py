@@ -6,8 +6,10 @@ include/chm_jacobi2d.h and include/charming_amd.h.  It loads the in-tree
 shared libraries built by charming_amd.build and raises if they are missing:
 there is no CPU fallback.
 """
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -107,6 +109,21 @@ class ServerStats(C.Structure):
 
 
 _jac = None
+# live handles, closed at interpreter exit (before the native libraries'
+# own exit handlers run) so no device state outlives the program
+_live = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live():
+    for obj in list(_live):
+        try:
+            if isinstance(obj, Server):
+                obj.stop(abort=True)
+            else:
+                obj.close()
+        except Exception:
+            pass
 
 
 def jacobi_lib():
@@ -165,6 +182,7 @@ class Jacobi2D:
         _check(L.chm_jacobi2d_create(C.byref(self.prob), self.FORMS[form],
                                      C.byref(h)))
         self.h = h
+        _live.add(self)
 
     def set_tuning(self, panel_chunks=0, rows_in_flight=0, band_rows=0,
                    wg_waves=0, overlap=0, depth=0, halo_copy=0):
@@ -297,6 +315,7 @@ class Server:
         h = C.c_void_p()
         _check(jacobi_lib().chm_jacobi2d_server_start(C.byref(cfg), C.byref(h)))
         self.h = h
+        _live.add(self)
         self._buf = (ServerCompletion * 256)()
 
     def submit(self, prob):

@@ -40,6 +40,7 @@ LIBS = [
     ("libchm_refnum.so", ["examples/refnum.hip"], ["charming_amd"]),
     ("libchm_pingpong.so", ["examples/pingpong.hip"], ["charming_amd"]),
     ("libchm_async_wait.so", ["examples/async_wait.hip"], ["charming_amd"]),
+    ("libchm_failure.so", ["examples/failure.hip"], ["charming_amd"]),
 ]
 
 # chare-program executables: the program's TU + the library-provided main
@@ -114,7 +115,7 @@ STACK_BYTES = 4096
 # chare programs (translation units that include <charming.h>)
 CHARE_PROGRAMS = ["jacobi2d/persistent.hip", "examples/hello.hip",
                   "examples/refnum.hip", "examples/pingpong.hip",
-                  "examples/async_wait.hip"]
+                  "examples/async_wait.hip", "examples/failure.hip"]
 
 
 def stack_need(src):

@@ -1,7 +1,10 @@
 // Hello chain: 2 x n_pes chares pass an incrementing integer along the chain
-// and the last one ends the run.  Same program shape as the reference's
-// examples/hello (hello.cu:6-52): delivery count and order are checked by
-// reading the printed lines.
+// and the last one ends the run.  Same program as the reference's
+// examples/hello (hello.cu:6-52), including its buffer contract: only
+// thread 0 allocates the integer it sends (a device `new`), every other
+// thread passes a null pointer, and invoke copies from thread 0's buffer
+// (get.cu:866-881).  Delivery count and order are checked by reading the
+// printed lines.
 #include <stdio.h>
 
 #include <charming.h>
@@ -26,23 +29,29 @@ __device__ void charm::main(int argc, char** argv, size_t* argvs, int pe) {
   barrier();
 
   if (pe == 0) {
-    __shared__ int first;
-    if (threadIdx.x == 0) first = 0;
+    int* send_int_p = nullptr;
+    if (threadIdx.x == 0) {
+      send_int_p = new int;
+      *send_int_p = 0;
+    }
     __syncthreads();
-    hello_proxy->invoke(0, 0, &first, sizeof(int));
+    hello_proxy->invoke(0, 0, send_int_p, sizeof(int));
   }
 }
 
 __device__ void Hello::greet(void* arg) {
-  const int got = *(int*)arg;
+  const int recv_int = *(int*)arg;
+  int* send_int_p = nullptr;
   if (threadIdx.x == 0)
-    printf("Hello I'm %d of %d! Received %d\n", i, n, got);
+    printf("Hello I'm %d of %d! Received %d\n", i, n, recv_int);
   if (i == n - 1) {
     charm::end();
   } else {
-    __shared__ int next;
-    if (threadIdx.x == 0) next = got + 1;
+    if (threadIdx.x == 0) {
+      send_int_p = new int;
+      *send_int_p = recv_int + 1;
+    }
     __syncthreads();
-    hello_proxy->invoke(i + 1, 0, &next, sizeof(int));
+    hello_proxy->invoke(i + 1, 0, send_int_p, sizeof(int));
   }
 }

@@ -111,6 +111,18 @@ void free_all() {
   r.inited = false;
 }
 
+// Process exit with the runtime still initialised (a host program that never
+// called chm_finalize): release the stream, events, pinned word and device
+// memory while the HIP runtime is still up.  Registered at the first
+// chm_init, so it runs before the HIP runtime's own exit handlers, which
+// were registered earlier (at the process's first HIP call).
+void runtime_atexit() {
+  Runtime& r = R();
+  if (!r.inited) return;
+  if (r.stream) (void)hipStreamSynchronize(r.stream);
+  free_all();
+}
+
 void default_config(chm_config* c) {
   memset(c, 0, sizeof(*c));
   c->ring_bytes = 8ull << 20;
@@ -127,6 +139,11 @@ void env_config(chm_config* c) {
   if (const char* s = getenv("CHM_RING_BYTES")) c->ring_bytes = strtoull(s, 0, 0);
   if (const char* s = getenv("CHM_VERBOSE")) c->verbose = atoi(s);
   if (const char* s = getenv("CHM_STACK_BYTES")) c->stack_bytes = strtoull(s, 0, 0);
+  if (const char* s = getenv("CHM_MAX_MSG_BYTES"))
+    c->max_msg_bytes = strtoull(s, 0, 0);
+  if (const char* s = getenv("CHM_DEFERRED_BYTES"))
+    c->deferred_bytes = strtoull(s, 0, 0);
+  if (const char* s = getenv("CHM_SPIN_LIMIT")) c->spin_limit = strtoull(s, 0, 0);
 }
 
 }  // namespace
@@ -184,6 +201,11 @@ int chm_init(int argc, char** argv, const chm_config* cfg_in) {
   if (cfg.device < 0 || cfg.device >= ndev)
     return chm::fail(CHM_E_INVALID, "device %d out of range", cfg.device);
   CHM_TRY_HIP(hipSetDevice(cfg.device));
+  static bool exit_hook = false;
+  if (!exit_hook) {
+    atexit(runtime_atexit);
+    exit_hook = true;
+  }
   // the scheduler kernel's stack is dynamic (entry methods are indirect
   // calls): its per-lane size comes from this limit, not from the compiler
   if (cfg.stack_bytes < 1024) cfg.stack_bytes = 1024;
@@ -242,7 +264,6 @@ int chm_init(int argc, char** argv, const chm_config* cfg_in) {
   CHM_STEP(dev_alloc(&h.deferred, (size_t)n_pes * cfg.deferred_bytes, false));
   CHM_STEP(dev_alloc(&h.heap, (size_t)n_pes * cfg.heap_bytes, false));
   CHM_STEP(dev_alloc(&h.ctl, charm::rt::CTL_WORDS * sizeof(uint32_t), true));
-  CHM_STEP(dev_alloc(&h.heartbeat, (size_t)n_pes * 128, true));
   CHM_STEP(dev_alloc(&r.stats_buf, (size_t)n_pes * sizeof(Stats), true));
   // argv in device memory (charming.cu:145-169 used managed memory)
   size_t total = 0;

@@ -138,7 +138,6 @@ struct Ctx {
   uint64_t heap_bytes;
   uint32_t* ctl;            // CTL_WORDS control words
   volatile uint32_t* host_abort;  // host-coherent word set by the watchdog
-  unsigned long long* heartbeat;  // per PE, stride 16
   uint64_t spin_limit;      // 0 = unbounded waits
   int argc;
   char** argv;

@@ -49,9 +49,20 @@ enum : int { CMD_EXIT = 0, CMD_INBOX = 1, CMD_REVIVED = 2, CMD_ASYNC = 3 };
 // Workgroup-shared scratch used to broadcast thread 0's decisions.  The
 // pending list holds publish words of messages already written but not yet
 // released: one L2 writeback then releases them all (see flush_publish).
+// Thread 0's arguments of a send: invoke is workgroup-collective, and as in
+// the reference (get.cu:866-881) only thread 0's buffer, size, destination
+// and header fields count; the other threads may pass anything (the
+// reference's hello passes a null buffer on every thread but 0).
+struct SendArgs {
+  const void* buf;
+  uint64_t payload;
+  int dst, type, chare_id, chare_idx, ep, refnum;
+};
+
 struct Lds {
   char* ptr;
   uint64_t pos;
+  SendArgs snd;
   uint64_t t_found;  // wall clock when the current message was found
   int cmd;
   int arg;
@@ -246,42 +257,58 @@ __device__ void queue_publish(int dst, uint64_t pos) {
   pe_state()->stats.sent++;
 }
 
-// Workgroup-collective: reserve a message in dst's ring and write its
-// headers; returns its address (nullptr on failure) and ring position.
+// Workgroup-collective: reserve a message in the destination's ring and
+// write its headers.  Thread 0's arguments decide (they are broadcast
+// through L.snd, which also keeps buf and payload for the caller's copy);
+// returns the message's address (nullptr on failure) and ring position.
 __device__ char* alloc_msg(int dst, msgtype type, int chare_id, int chare_idx,
-                           int ep, int refnum, size_t payload, uint64_t* pos_out) {
+                           int ep, int refnum, const void* buf, size_t payload,
+                           uint64_t* pos_out) {
   const Ctx& c = g_ctx;
   Lds& L = lds();
-  const uint64_t alloc = envelope::alloc_size(type, payload);
-  const uint64_t slot = (alloc + kGranule - 1) & ~(kGranule - 1);
   __syncthreads();
-  if (threadIdx.x == 0) L.pos = reserve(dst, slot);
+  if (threadIdx.x == 0) {
+    L.snd.buf = buf;
+    L.snd.payload = payload;
+    L.snd.dst = dst;
+    L.snd.type = (int)type;
+    L.snd.chare_id = chare_id;
+    L.snd.chare_idx = chare_idx;
+    L.snd.ep = ep;
+    L.snd.refnum = refnum;
+    const uint64_t alloc = envelope::alloc_size(type, payload);
+    L.pos = reserve(dst, (alloc + kGranule - 1) & ~(kGranule - 1));
+    if (L.pos != kFail) {
+      char* base = c.rings + (uint64_t)dst * c.ring_stride +
+                   (L.pos & (c.ring_bytes - 1));
+      new (base) envelope(type, alloc);
+      new (base + sizeof(envelope))
+          regular_msg(chare_id, chare_idx, ep, refnum);
+    }
+  }
   __syncthreads();
   const uint64_t pos = L.pos;
   *pos_out = pos;
   if (pos == kFail) return nullptr;
-  char* base = c.rings + (uint64_t)dst * c.ring_stride +
-               (pos & (c.ring_bytes - 1));
-  if (threadIdx.x == 0) {
-    new (base) envelope(type, alloc);
-    new (base + sizeof(envelope))
-        regular_msg(chare_id, chare_idx, ep, refnum);
-  }
-  return base;
+  return c.rings + (uint64_t)L.snd.dst * c.ring_stride +
+         (pos & (c.ring_bytes - 1));
 }
 
-// Workgroup-collective send of one message to PE dst.
+// Workgroup-collective send of one message (thread 0's arguments).
 __device__ void send_msg(int dst, msgtype type, int chare_id, int chare_idx,
                          int ep, int refnum, const void* buf, size_t payload) {
   uint64_t pos = 0;
-  char* base = alloc_msg(dst, type, chare_id, chare_idx, ep, refnum, payload,
-                         &pos);
+  char* base = alloc_msg(dst, type, chare_id, chare_idx, ep, refnum, buf,
+                         payload, &pos);
   if (!base) return;
-  if (payload) wg_copy(base + sizeof(envelope) + sizeof(regular_msg),
-                       (const char*)buf, payload);
+  Lds& L = lds();
+  const uint64_t n = L.snd.payload;
+  const int to = L.snd.dst;
+  if (n) wg_copy(base + sizeof(envelope) + sizeof(regular_msg),
+                 (const char*)L.snd.buf, n);
   wave_drain();
   __syncthreads();
-  if (threadIdx.x == 0) queue_publish(dst, pos);
+  if (threadIdx.x == 0) queue_publish(to, pos);
 }
 
 // ---------------------------------------------------------------------------
@@ -643,10 +670,10 @@ __device__ void* alloc_user_msg(message& msg, int chare_id, int chare_idx,
                                 int refnum) {
   uint64_t pos = 0;
   char* base = rt::alloc_msg(dst_pe, msgtype::user, chare_id, chare_idx, ep_id,
-                             refnum, payload_size, &pos);
+                             refnum, nullptr, payload_size, &pos);
   msg.env = (envelope*)base;
   msg.offset = pos;
-  msg.dst_pe = dst_pe;
+  msg.dst_pe = rt::lds().snd.dst;
   return msg.payload();
 }
 

@@ -49,18 +49,54 @@ def test_reference_init_known_answers():
     assert g2[2, 2] == (0.0 + 2.0 + 0.0 + 2.0 + 0.0) * 0.2 == 0.8
 
 
-def test_operand_order_and_multiply_by_point_two():
-    # ((((c + W) + E) + N) + S) * 0.2 with values where any other order differs
+# Four stencils whose five operands mix 1e16 (ulp 2) with small values, so
+# that the rounding of every partial sum depends on the order of the
+# additions.  Expected values are worked by hand for the reference order
+# ((((c + W) + E) + N) + S) * 0.2 (jacobi2d.cu:332-333), e.g. the third:
+# 0.75 + 0.5 = 1.25; 1.25 + 1e16 rounds to 1e16 + 2; + 1.0 = 1e16 + 3 ties to
+# the even 1e16 + 4; - 1e16 = 4; 4 * 0.2 = 0.8.
+ORDER_KATS = [  # (c, W, E, N, S) -> result
+    ((1e16, 1.0, -1e16, 0.75, 0.5), 0.25),
+    ((1.0, 1e16, 0.75, -1e16, 0.5), 0.1),
+    ((0.75, 0.5, 1e16, 1.0, -1e16), 0.8),
+    ((-1e16, 0.75, 1.0, 1e16, 0.5), 0.1),
+]
+
+
+def _stencil_at_center(c, w, e, n, s):
     g = np.zeros((3, 3))
-    g[1, 1], g[1, 0], g[1, 2], g[0, 1], g[2, 1] = 1e16, 1.0, 1.0, -1e16, 0.0
-    assert O.step(g)[1, 1] == 0.0           # c+N first would give 0.4
-    g[:] = 0.0
-    g[1, 1], g[1, 0], g[1, 2] = 1.0, 1.0, 1.0   # sum 3
-    assert O.step(g)[1, 1] == 0.6000000000000001  # 3*0.2, not 3/5 == 0.6
-    g[:] = 0.0
-    g[1, 1], g[0, 1] = 1e16, 1.0      # 1e16+... + N(1.0) rounds away, S=1
-    g[2, 1] = 1.0
-    assert O.step(g)[1, 1] == 1e16 * 0.2
+    g[1, 1], g[1, 0], g[1, 2], g[0, 1], g[2, 1] = c, w, e, n, s
+    return O.step(g)[1, 1]
+
+
+@pytest.mark.parametrize("vals,expected", ORDER_KATS)
+def test_operand_order_known_answers(vals, expected):
+    assert _stencil_at_center(*vals) == expected
+
+
+def test_order_kats_separate_every_other_order():
+    # The vectors above pin the order only if every other left-to-right
+    # order of the five operands (apart from swapping the first two, which
+    # commute) gives a different result on at least one of them.
+    import itertools
+    ref = (0, 1, 2, 3, 4)
+    for p in itertools.permutations(range(5)):
+        if p[2:] == ref[2:]:
+            continue  # the reference order, or its first two swapped
+        differs = False
+        for vals, expected in ORDER_KATS:
+            acc = np.float64(vals[p[0]])
+            for k in p[1:]:
+                acc = acc + np.float64(vals[k])
+            differs |= bool(acc * np.float64(0.2) != expected)
+        assert differs, p
+
+
+def test_multiply_by_point_two_not_divide_by_five():
+    # sum 3: 3 * 0.2 = 0.6000000000000001, while 3 / 5 = 0.6
+    assert _stencil_at_center(1.0, 1.0, 1.0, 0.0, 0.0) == 0.6000000000000001
+    # 1e16 + 1 + 1 rounds back to 1e16 at each step (ulp 2, ties to even)
+    assert _stencil_at_center(1e16, 0.0, 0.0, 1.0, 1.0) == 1e16 * 0.2
 
 
 @pytest.mark.parametrize("W,H,cr,cc,iters,pes", [
@@ -87,9 +123,9 @@ def test_residual_closed_form():
     assert O.residual_fixed(a, b) == 200 * 2**54
     b = a.copy()
     b[1:-1, 1:-1] = a[1:-1, 1:-1] + 2.0**-30 * (a[1:-1, 1:-1] * 0 + 1)
-    # differences are tiny; floor truncates every term to 0 or a small integer
+    # d = 2^-30 exactly (dyadic a): d^2 * 2^56 = 2^-4, which floors to 0
     s = O.residual_fixed(a, b)
-    assert 0 <= s <= 200
+    assert s == 0
     c = np.zeros((3, 3)); d = np.zeros((3, 3)); d[1, 1] = 10.0
     assert O.residual_fixed(c, d) == 100 * 2**56    # max possible term
     assert O.u128_to_double(2**64 + 5) == 2.0**64 + 5.0
