@@ -41,6 +41,10 @@ LIBS = [
     ("libchm_pingpong.so", ["examples/pingpong.hip"], ["charming_amd"]),
     ("libchm_async_wait.so", ["examples/async_wait.hip"], ["charming_amd"]),
     ("libchm_failure.so", ["examples/failure.hip"], ["charming_amd"]),
+    # GPU-level builds (one PE spanning every workgroup): SRC@gpu is SRC
+    # compiled with CHARMING_GPU_LEVEL
+    ("libchm_hello_gpu.so", ["examples/hello.hip@gpu"], ["charming_amd"]),
+    ("libchm_pingpong_gpu.so", ["examples/pingpong.hip@gpu"], ["charming_amd"]),
 ]
 
 # chare-program executables: the program's TU + the library-provided main
@@ -49,6 +53,8 @@ EXES = [
     ("refnum", ["examples/refnum.hip"]),
     ("pingpong", ["examples/pingpong.hip"]),
     ("async_wait", ["examples/async_wait.hip"]),
+    ("hello_gpu", ["examples/hello.hip@gpu"]),
+    ("pingpong_gpu", ["examples/pingpong.hip@gpu"]),
 ]
 MAIN_SRC = "runtime/main.cpp"
 # standalone programs (their own main, no runtime): the raw mailbox pingpong
@@ -60,12 +66,19 @@ BINDIR = os.path.join(PKG, "bin")
 
 
 def _obj(src):
-    return os.path.join(OBJDIR, src.replace("/", "__") + ".o")
+    return os.path.join(OBJDIR, src.replace("/", "__").replace("@", "__") + ".o")
+
+
+def _split(src):
+    """'path@gpu' -> (path, extra flags of the GPU-level build)."""
+    path, _, variant = src.partition("@")
+    return path, (["-DCHARMING_GPU_LEVEL"] if variant == "gpu" else [])
 
 
 def _compile(src, verbose):
-    path = os.path.join(CSRC, src)
-    cmd = [HIPCC] + DEVICE_FLAGS + ["-c", path, "-o", _obj(src)]
+    rel, extra = _split(src)
+    path = os.path.join(CSRC, rel)
+    cmd = [HIPCC] + DEVICE_FLAGS + extra + ["-c", path, "-o", _obj(src)]
     if verbose:
         print(" ".join(cmd), flush=True)
     p = subprocess.run(cmd, capture_output=True, text=True)
@@ -115,7 +128,8 @@ STACK_BYTES = 4096
 # chare programs (translation units that include <charming.h>)
 CHARE_PROGRAMS = ["jacobi2d/persistent.hip", "examples/hello.hip",
                   "examples/refnum.hip", "examples/pingpong.hip",
-                  "examples/async_wait.hip", "examples/failure.hip"]
+                  "examples/async_wait.hip", "examples/failure.hip",
+                  "examples/hello.hip@gpu", "examples/pingpong.hip@gpu"]
 
 
 def stack_need(src):
@@ -126,10 +140,11 @@ def stack_need(src):
     cannot size it).  Computed from the frame sizes in the device assembly."""
     import re
     import tempfile
+    rel, extra = _split(src)
     with tempfile.TemporaryDirectory() as d:
         out = os.path.join(d, "k.s")
-        cmd = [HIPCC] + DEVICE_FLAGS + ["-S", "--cuda-device-only",
-                                        os.path.join(CSRC, src), "-o", out]
+        cmd = [HIPCC] + DEVICE_FLAGS + extra + ["-S", "--cuda-device-only",
+                                                os.path.join(CSRC, rel), "-o", out]
         p = subprocess.run(cmd, capture_output=True, text=True)
         if p.returncode != 0:
             raise RuntimeError(f"hipcc -S failed on {src}:\n{p.stderr}")
@@ -156,7 +171,8 @@ def build(verbose=False, jobs=None):
     srcs = sorted({s for _, ss, _ in LIBS for s in ss} |
                   {s for _, ss in EXES for s in ss} |
                   {s for _, ss in STANDALONE_EXES for s in ss} | {MAIN_SRC})
-    missing = [s for s in srcs if not os.path.exists(os.path.join(CSRC, s))]
+    missing = [s for s in srcs
+               if not os.path.exists(os.path.join(CSRC, _split(s)[0]))]
     if missing:
         raise FileNotFoundError(f"missing sources: {missing}")
     jobs = jobs or min(8, os.cpu_count() or 4)

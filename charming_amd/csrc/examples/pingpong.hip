@@ -1,9 +1,18 @@
-// Pingpong between two chares on PE 0 and PE n_pes/2 (the reference's
-// examples/pingpong, pingpong.cu:17-217): payload sizes double from min to
-// max, `iters` timed round trips after `warmup`, one-way latency =
-// elapsed / 2 / iters.  Added here: payloads are seeded bytes that the
-// receiver checks, and every message carries refnum = its round-trip
-// sequence number, which the receiver's refnum gate must match exactly.
+// Pingpong between two chares (the reference's examples/pingpong,
+// pingpong.cu:17-217): payload sizes double from min to max, `iters` timed
+// round trips after `warmup`, one-way latency = elapsed / 2 / iters.  Added
+// here: payloads are seeded bytes that the receiver checks, and every
+// message carries refnum = its round-trip sequence number, which the
+// receiver's refnum gate must match exactly.
+//
+// Like the reference it builds in both modes:
+//   one PE per workgroup (default, the reference's SM_LEVEL): chares on PE 0
+//   and PE n_pes/2, entry methods run by one workgroup;
+//   GPU-level (built with CHARMING_GPU_LEVEL, the reference's default build,
+//   pingpong.cu:9-15): one PE spans every workgroup, so on one GPU both
+//   chares live on PE 0 and their messages go through its own inbox; entry
+//   methods and invoke run on every thread of the launch, and the payload
+//   buffers come from malloc_user.
 //
 // argv: [min_size [max_size [iters [warmup [check]]]]]
 #include <hip/hip_runtime.h>
@@ -12,13 +21,25 @@
 #include <charming.h>
 #include <charming_chrono.h>
 
+#ifdef SM_LEVEL
+#define PP_TID (threadIdx.x)
+#define PP_NT (blockDim.x)
+#define BARRIER_LOCAL __syncthreads()
+#else
+#define PP_TID (GID)
+#define PP_NT (gridDim.x * blockDim.x)
+#define BARRIER_LOCAL charm::barrier_local()
+#endif
+
 struct Comm : charm::chare {
   int index, peer, init_cnt;
   size_t min_size, max_size, cur_size;
   int n_iters, warmup, iter, check;
   int seq;             // round trips started (chare 0) / answered (chare 1)
+  int finished;
   unsigned long long errors;
   char* data;
+  size_t data_off;     // GPU-level: offset of data in the user arena
   charm::chrono::time_point<charm::chrono::system_clock> start_tp;
 
   __device__ Comm() {}
@@ -32,6 +53,7 @@ __device__ void entry_init(Comm& c, void* arg) { c.init(arg); }
 __device__ void entry_init_done(Comm& c, void* arg) { c.init_done(arg); }
 __device__ void entry_recv(Comm& c, void* arg) { c.recv(arg); }
 
+#ifdef SM_LEVEL
 __shared__ charm::chare_proxy<Comm>* comm_proxy;
 __device__ char* g_pp_buffers[2];  // set by main_host
 
@@ -44,6 +66,11 @@ void charm::main_host(int argc, char** argv) {
     if (hipMalloc(&bufs[i], g_host_max + 64) != hipSuccess) bufs[i] = nullptr;
   (void)hipMemcpyToSymbol(HIP_SYMBOL(g_pp_buffers), bufs, sizeof(bufs));
 }
+#else
+__device__ charm::chare_proxy<Comm>* comm_proxy;
+__device__ size_t g_params[5];
+__device__ unsigned long long g_bad;
+#endif
 
 // Byte k of a size-s payload travelling in direction dir.
 __device__ inline unsigned char pattern(size_t k, size_t s, int dir) {
@@ -54,24 +81,33 @@ __device__ inline unsigned char pattern(size_t k, size_t s, int dir) {
 }
 
 __device__ void fill(char* p, size_t s, int dir) {
-  for (size_t k = threadIdx.x; k < s; k += blockDim.x) p[k] = (char)pattern(k, s, dir);
-  __syncthreads();
+  for (size_t k = PP_TID; k < s; k += PP_NT) p[k] = (char)pattern(k, s, dir);
+  BARRIER_LOCAL;
 }
 
+// Mismatched bytes of a received payload (collective).
 __device__ unsigned long long verify(const char* p, size_t s, int dir) {
+  unsigned long long mine = 0;
+  for (size_t k = PP_TID; k < s; k += PP_NT)
+    mine += ((unsigned char)p[k] != pattern(k, s, dir));
+#ifdef SM_LEVEL
   __shared__ unsigned long long bad;
   if (threadIdx.x == 0) bad = 0;
   __syncthreads();
-  unsigned long long mine = 0;
-  for (size_t k = threadIdx.x; k < s; k += blockDim.x)
-    mine += ((unsigned char)p[k] != pattern(k, s, dir));
   if (mine) atomicAdd(&bad, mine);
   __syncthreads();
   return bad;
+#else
+  if (GID == 0) g_bad = 0;
+  BARRIER_LOCAL;
+  if (mine) atomicAdd(&g_bad, mine);
+  BARRIER_LOCAL;
+  return __hip_atomic_load(&g_bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#endif
 }
 
 __device__ void charm::main(int argc, char** argv, size_t* argvs, int pe) {
-  if (threadIdx.x == 0) {
+  if (PP_TID == 0) {
     comm_proxy = new charm::chare_proxy<Comm>();
     comm_proxy->add_entry_method<&entry_init>();
     comm_proxy->add_entry_method<&entry_init_done>();
@@ -84,11 +120,15 @@ __device__ void charm::main(int argc, char** argv, size_t* argvs, int pe) {
     comm_proxy->create(2, block_map);
     delete[] block_map;
   }
-  __syncthreads();
+  BARRIER_LOCAL;
   barrier();
   if (pe == 0) {
+#ifdef SM_LEVEL
     __shared__ size_t params[5];
-    if (threadIdx.x == 0) {
+#else
+    size_t* params = g_params;
+#endif
+    if (PP_TID == 0) {
       params[0] = 8;
       params[1] = 1048576;
       params[2] = 1000;
@@ -100,13 +140,13 @@ __device__ void charm::main(int argc, char** argv, size_t* argvs, int pe) {
              charm::n_pes(), (unsigned long long)params[0],
              (unsigned long long)params[1], (int)params[2], (int)params[3]);
     }
-    __syncthreads();
-    comm_proxy->invoke_all(0, params, sizeof(params));
+    BARRIER_LOCAL;
+    comm_proxy->invoke_all(0, params, sizeof(size_t) * 5);
   }
 }
 
 __device__ void Comm::init(void* arg) {
-  if (threadIdx.x == 0) {
+  if (PP_TID == 0) {
     const size_t* p = (const size_t*)arg;
     min_size = p[0];
     max_size = p[1];
@@ -118,42 +158,47 @@ __device__ void Comm::init(void* arg) {
     seq = 0;
     errors = 0;
     init_cnt = 0;
+    finished = 0;
     index = i;
     peer = (index == 0) ? 1 : 0;
+#ifdef SM_LEVEL
     data = g_pp_buffers[index];
+#else
+    data = (char*)charm::malloc_user(max_size, data_off);
+    if (!data) charm::abort();
+#endif
     comm_proxy->set_refnum(index, 0);
   }
-  __syncthreads();
+  BARRIER_LOCAL;
   comm_proxy->invoke(0, 1);
 }
 
 __device__ void Comm::init_done(void* arg) {
-  if (threadIdx.x == 0) init_cnt++;
-  __syncthreads();
+  if (PP_TID == 0) init_cnt++;
+  BARRIER_LOCAL;
   if (init_cnt == 2) send();
 }
 
 __device__ void Comm::send() {
   // only chare 0 starts round trips
   if (iter == 0) fill(data, cur_size, 0);
-  if (threadIdx.x == 0 && iter == warmup)
+  if (PP_TID == 0 && iter == warmup)
     start_tp = charm::chrono::system_clock::now();
-  __syncthreads();
+  BARRIER_LOCAL;
   comm_proxy->invoke(peer, 2, data, cur_size, seq);
 }
 
 __device__ void Comm::recv(void* arg) {
-  __shared__ int finished;
   const size_t sz = cur_size;  // size of the message being handled
   const int my_seq = seq;
   const int dir_in = (index == 0) ? 1 : 0;
-  __syncthreads();
+  BARRIER_LOCAL;
   const unsigned long long bad =
       check ? verify((const char*)arg, sz, dir_in) : 0;
   if (index == 1) {
     // echo side: answer with this chare's own seeded buffer of the same size
     if (iter == 0) fill(data, sz, 1);
-    if (threadIdx.x == 0) {
+    if (PP_TID == 0) {
       errors += bad;
       seq++;
       comm_proxy->set_refnum(index, seq);
@@ -164,13 +209,17 @@ __device__ void Comm::recv(void* arg) {
         if (cur_size > max_size) finished = 1;
       }
     }
-    __syncthreads();
+    BARRIER_LOCAL;
     comm_proxy->invoke(peer, 2, data, sz, my_seq);
-    if (finished && threadIdx.x == 0)
+    if (finished && PP_TID == 0) {
       printf("Pingpong peer done: payload errors %llu\n", errors);
+#ifndef SM_LEVEL
+      charm::free_user(max_size, data_off);
+#endif
+    }
     return;
   }
-  if (threadIdx.x == 0) {
+  if (PP_TID == 0) {
     errors += bad;
     finished = 0;
     seq++;
@@ -185,11 +234,15 @@ __device__ void Comm::recv(void* arg) {
       if (cur_size > max_size) finished = 1;
     }
   }
-  __syncthreads();
+  BARRIER_LOCAL;
   if (finished) {
-    if (threadIdx.x == 0)
+    if (PP_TID == 0) {
       printf("Pingpong done: round trips %d, payload errors %llu\n", seq,
              errors);
+#ifndef SM_LEVEL
+      charm::free_user(max_size, data_off);
+#endif
+    }
     charm::end();
   } else {
     send();

@@ -38,6 +38,7 @@ struct Runtime {
   std::vector<void*> dev_allocs;
   Stats* stats_buf = nullptr;  // n_pes packed statistics (chm_get_stats)
   chm_stats last;
+  int grid_wgs = 0;  // workgroups launched (n_pes, or the GPU-level PE's)
 };
 
 // Packs the statistics of every PE (one field of each PeState) into one
@@ -130,6 +131,7 @@ void default_config(chm_config* c) {
   c->deferred_bytes = 2ull << 20;
   c->heap_bytes = 1ull << 20;
   c->stack_bytes = 4096;
+  c->user_bytes = 64ull << 20;
 }
 
 // Environment overrides for programs started through the library main.
@@ -144,6 +146,8 @@ void env_config(chm_config* c) {
   if (const char* s = getenv("CHM_DEFERRED_BYTES"))
     c->deferred_bytes = strtoull(s, 0, 0);
   if (const char* s = getenv("CHM_SPIN_LIMIT")) c->spin_limit = strtoull(s, 0, 0);
+  if (const char* s = getenv("CHM_GPU_WGS")) c->workgroups = atoi(s);
+  if (const char* s = getenv("CHM_USER_BYTES")) c->user_bytes = strtoull(s, 0, 0);
 }
 
 }  // namespace
@@ -162,10 +166,17 @@ int chm_register_program(const chm_program* prog) {
 
 int chm_select_program(const char* name_part) {
   Runtime& r = R();
-  for (int i = (int)r.programs.size() - 1; i >= 0; i--)
-    if (name_part && strstr(r.programs[i].name, name_part)) {
-      r.selected = i;
-      return CHM_OK;
+  const size_t np = name_part ? strlen(name_part) : 0;
+  for (int pass = 0; pass < 2 && name_part; pass++)
+    for (int i = (int)r.programs.size() - 1; i >= 0; i--) {
+      const char* nm = r.programs[i].name;
+      const size_t nl = strlen(nm);
+      const bool hit = pass == 0 ? (nl >= np && strcmp(nm + nl - np, name_part) == 0)
+                                 : strstr(nm, name_part) != nullptr;
+      if (hit) {
+        r.selected = i;
+        return CHM_OK;
+      }
     }
   return chm::fail(CHM_E_INVALID, "no registered program matches '%s'",
                    name_part ? name_part : "(null)");
@@ -192,6 +203,8 @@ int chm_init(int argc, char** argv, const chm_config* cfg_in) {
     cfg.spin_limit = cfg_in->spin_limit;
     cfg.verbose = cfg_in->verbose;
     if (cfg_in->stack_bytes) cfg.stack_bytes = cfg_in->stack_bytes;
+    cfg.workgroups = cfg_in->workgroups;
+    if (cfg_in->user_bytes) cfg.user_bytes = cfg_in->user_bytes;
   } else {
     env_config(&cfg);
   }
@@ -219,12 +232,20 @@ int chm_init(int argc, char** argv, const chm_config* cfg_in) {
   int per_cu = 0;
   CHM_TRY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
       &per_cu, r.prog.kernel, threads, 0));
-  const int n_pes = cfg.n_pes > 0 ? cfg.n_pes : cus;
-  if (per_cu < 1 || n_pes > per_cu * cus)
+  // workgroup-level programs: one workgroup per PE; GPU-level programs: one
+  // PE (this GPU, as the reference's c_n_pes on one device) spanning
+  // `workgroups` workgroups
+  const bool gpu_level = r.prog.gpu_level != 0;
+  const int n_pes = gpu_level ? 1 : (cfg.n_pes > 0 ? cfg.n_pes : cus);
+  const int grid_wgs =
+      gpu_level ? (cfg.workgroups > 0 ? cfg.workgroups : cus) : n_pes;
+  if (per_cu < 1 || grid_wgs > per_cu * cus)
     return chm::fail(CHM_E_INVALID,
-                     "%d PEs of %d threads exceed the co-resident capacity "
-                     "(%d per CU x %d CUs)",
-                     n_pes, threads, per_cu, cus);
+                     "%d workgroups of %d threads exceed the co-resident "
+                     "capacity (%d per CU x %d CUs)",
+                     grid_wgs, threads, per_cu, cus);
+  r.grid_wgs = grid_wgs;
+  cfg.user_bytes = (cfg.user_bytes + 255) / 256 * 256;
   if (cfg.ring_bytes < (64u << 10)) cfg.ring_bytes = 64u << 10;
   cfg.ring_bytes = round_pow2(cfg.ring_bytes);
   cfg.max_msg_bytes =
@@ -249,6 +270,7 @@ int chm_init(int argc, char** argv, const chm_config* cfg_in) {
   h.deferred_bytes = cfg.deferred_bytes;
   h.heap_bytes = cfg.heap_bytes;
   h.spin_limit = cfg.spin_limit;
+  h.user_bytes = gpu_level ? cfg.user_bytes : 0;
   int rc = CHM_OK;
   const size_t ng = cfg.ring_bytes / charm::rt::kGranule;
 #define CHM_STEP(x)       \
@@ -265,6 +287,9 @@ int chm_init(int argc, char** argv, const chm_config* cfg_in) {
   CHM_STEP(dev_alloc(&h.heap, (size_t)n_pes * cfg.heap_bytes, false));
   CHM_STEP(dev_alloc(&h.ctl, charm::rt::CTL_WORDS * sizeof(uint32_t), true));
   CHM_STEP(dev_alloc(&r.stats_buf, (size_t)n_pes * sizeof(Stats), true));
+  CHM_STEP(dev_alloc(&h.bcast, charm::rt::kBcastWords * sizeof(uint64_t), true));
+  if (gpu_level)
+    CHM_STEP(dev_alloc(&h.user_arena, (size_t)n_pes * cfg.user_bytes, false));
   // argv in device memory (charming.cu:145-169 used managed memory)
   size_t total = 0;
   for (int i = 0; i < argc; i++) total += strlen(argv[i]) + 1;
@@ -365,7 +390,7 @@ int chm_run(void) {
   void* args[] = {&dummy};
   CHM_TRY_HIP(hipEventRecord(r.ev0, r.stream));
   CHM_TRY_HIP(hipLaunchCooperativeKernel(
-      r.prog.kernel, dim3(r.h.n_pes), dim3(r.h.threads_per_pe), args, 0,
+      r.prog.kernel, dim3(r.grid_wgs), dim3(r.h.threads_per_pe), args, 0,
       r.stream));
   CHM_TRY_HIP(hipEventRecord(r.ev1, r.stream));
   // watchdog (no counterpart in the reference, which blocks forever)
@@ -459,10 +484,11 @@ int chm_main(int argc, char** argv) {
     chm_stats s;
     if (st && atoi(st) && chm_get_stats(&s) == CHM_OK) {
       fflush(stdout);
-      printf("[STATS] pes=%d threads=%d delivered=%llu sent=%llu deferred=%llu "
-             "revived=%llu send_spins=%llu kernel_ms=%.3f overhead_ms=%.3f "
-             "error=%u\n",
-             s.n_pes, s.threads_per_pe, s.delivered, s.sent, s.deferred,
+      printf("[STATS] pes=%d wgs=%d threads=%d delivered=%llu sent=%llu "
+             "deferred=%llu revived=%llu send_spins=%llu kernel_ms=%.3f "
+             "overhead_ms=%.3f error=%u\n",
+             s.n_pes, r.grid_wgs, s.threads_per_pe, s.delivered, s.sent,
+             s.deferred,
              s.revived, s.send_spins, s.kernel_ms, s.overhead_ms_per_pe,
              s.error_code);
     }

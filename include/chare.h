@@ -131,6 +131,20 @@ struct chare_proxy_base {
     rt::raise_error(rt::ERR_WAITS_FULL, (uint32_t)ASYNC_WAIT_MAX);
   }
 
+  // The refnum gate of call() on its own (GPU-level mode, where the leader
+  // decides and then every workgroup runs the entry method): 1 = run now,
+  // 0 = defer, -1 = bad chare or entry method (an error is raised).
+  __device__ int gate(int idx, int ep, int ref) const {
+    if (idx < start_idx || idx > end_idx || ep < 0 || ep >= em_count) {
+      rt::raise_error(rt::ERR_BAD_CHARE, (uint32_t)idx);
+      return -1;
+    }
+    return (ref == -1 || ref == refnum[idx - start_idx]) ? 1 : 0;
+  }
+  __device__ void run(int idx, int ep, void* arg) {
+    entries[ep](objs[idx - start_idx], arg);
+  }
+
   // Refnum gate then entry method (chare.h:181-195).  Workgroup-collective.
   // Returns false when the message must be deferred.
   __device__ bool call(int idx, int ep, void* arg, int ref) {

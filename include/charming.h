@@ -16,7 +16,8 @@ namespace charm {
 // Optional: a program that does not define it gets none.
 __attribute__((weak)) void main_host(int argc, char** argv);
 
-// Main function executed on all PEs (by every thread of each PE's workgroup)
+// Main function executed on all PEs (by every thread of each PE's workgroup,
+// or of every workgroup in GPU-level mode)
 __device__ void main(int argc, char** argv, size_t* argvs, int pe);
 
 // Runtime functions that can be called by the user
@@ -34,6 +35,13 @@ __device__ void barrier();
 __device__ void barrier_local();
 
 __device__ int device_atoi(const char* str, int strlen);
+
+#ifndef SM_LEVEL
+// GPU-level mode only (charming.h:31-34 of the reference): memory from the
+// PE's user arena; single-threaded.
+__device__ void* malloc_user(size_t size, size_t& offset);
+__device__ void free_user(size_t size, size_t offset);
+#endif
 
 // Seconds on the device wall clock (constant rate).
 __device__ double wall_time();

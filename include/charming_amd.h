@@ -40,6 +40,10 @@ typedef struct chm_config {
                                compiler cannot size the stack of the chain below
                                them; the runtime's default of 1 KiB is too small
                                for some entry methods */
+  int workgroups;           /* GPU-level programs: workgroups of the one PE
+                               (0 = one per CU); ignored otherwise */
+  size_t user_bytes;        /* GPU-level programs: per-PE arena of malloc_user
+                               (0 = 64 MiB) */
 } chm_config;
 
 typedef struct chm_stats {
@@ -64,6 +68,9 @@ typedef struct chm_program {
   void (*main_host)(int, char**);/* charm::main_host, or NULL if not defined */
   int (*set_ctx)(const void* ctx, size_t bytes); /* copies Ctx to the TU's symbol */
   const char* name;
+  int gpu_level;                 /* 1: one PE spans every workgroup (built
+                                    with CHARMING_GPU_LEVEL, the reference's
+                                    build without SM_LEVEL) */
 } chm_program;
 
 /* Called at load time by every program that includes <charming.h>;
@@ -71,8 +78,10 @@ typedef struct chm_program {
  * linked into libcharming.a with relocatable device code, Makefile:38). */
 int chm_register_program(const chm_program* prog);
 
-/* Choose among several registered programs by a substring of the source
- * file name they were registered with (default: the last one loaded). */
+/* Choose among several registered programs by the source file name they
+ * were registered with (default: the last one loaded): the latest program
+ * whose name ends with name_part, else the latest that contains it.
+ * GPU-level programs are registered as "<source> (gpu-level)". */
 int chm_select_program(const char* name_part);
 
 /* charming.cu:76-235: pick the device, check cooperative launch, size and

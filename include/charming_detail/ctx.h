@@ -105,8 +105,17 @@ struct AsyncWait {
   int valid;
 };
 
+// A free extent of the PE's user arena (malloc_user / free_user).
+struct UserExtent {
+  uint64_t offset;
+  uint64_t size;
+};
+constexpr int kUserExtents = 64;
+
 struct PeState {
   ProxyTable table;
+  int n_user_free;     // free extents of the user arena, sorted by offset
+  UserExtent user_free[kUserExtents];
   int n_waits;         // valid entries of waits[]
   AsyncWait waits[ASYNC_WAIT_MAX];
   uint64_t read_pos;   // logical position of the next inbox message
@@ -144,7 +153,14 @@ struct Ctx {
   size_t* argvs;
   void* user;               // opaque pointer for the application
   double wall_hz;           // frequency of the device wall clock
+  // GPU-level mode (one PE spanning every workgroup of the launch)
+  uint64_t* bcast;          // leader -> all workgroups: decisions of the
+                            // scheduler and of sends (kBcastWords words)
+  char* user_arena;         // n_pes x user_bytes (malloc_user)
+  uint64_t user_bytes;
 };
+
+constexpr int kBcastWords = 32;
 
 }  // namespace rt
 }  // namespace charm

@@ -63,6 +63,9 @@ struct Lds {
   char* ptr;
   uint64_t pos;
   SendArgs snd;
+  uint64_t g_slot;   // GPU-level leader: ring bytes of the current message
+  int g_kind;        // GPU-level leader: what poll_next found (CMD_*)
+  int g_run;         // GPU-level leader: the entry method runs (1) or not
   uint64_t t_found;  // wall clock when the current message was found
   int cmd;
   int arg;
@@ -73,6 +76,19 @@ struct Lds {
   uint32_t order[CHARMING_DEFERRED_MAX];
 };
 static __shared__ Lds s_lds;
+
+#ifdef CHARMING_GPU_LEVEL
+// GPU-level mode (the reference's build without SM_LEVEL, scheduler.cu:
+// 240-333, get.cu:990-1313): one PE spans every workgroup of the launch.
+// Its leader (workgroup 0, thread 0) polls the inbox, decides the refnum
+// gate and reserves messages; the decisions reach the other workgroups
+// through g_ctx.bcast and a grid barrier; every thread of every workgroup
+// runs main, the entry methods and invoke.
+__device__ inline int pe_index() { return 0; }
+#else
+// One PE per workgroup (the reference's SM_LEVEL mode).
+__device__ inline int pe_index() { return (int)blockIdx.x; }
+#endif
 
 // ---------------------------------------------------------------------------
 // memory-model helpers (agent scope = the whole GPU, all 8 XCDs)
@@ -115,7 +131,7 @@ __device__ inline void wave_drain() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
-__device__ PeState* pe_state() { return g_ctx.pes + blockIdx.x; }
+__device__ PeState* pe_state() { return g_ctx.pes + pe_index(); }
 __device__ inline Lds& lds() { return s_lds; }
 __device__ inline uint64_t n_granules() { return g_ctx.ring_bytes / kGranule; }
 
@@ -125,7 +141,7 @@ __device__ void raise_error(uint32_t code, uint32_t aux) {
   if (__hip_atomic_compare_exchange_strong(&ctl[CTL_ERROR], &expected, code,
                                            __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT)) {
-    __hip_atomic_store(&ctl[CTL_ERROR_PE], (uint32_t)blockIdx.x,
+    __hip_atomic_store(&ctl[CTL_ERROR_PE], (uint32_t)pe_index(),
                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&ctl[CTL_ERROR_AUX], aux, __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
@@ -146,7 +162,7 @@ __device__ void* heap_alloc(size_t bytes, size_t align) {
     return nullptr;
   }
   ps->heap_used = off + bytes;
-  return g_ctx.heap + (uint64_t)blockIdx.x * g_ctx.heap_bytes + off;
+  return g_ctx.heap + (uint64_t)pe_index() * g_ctx.heap_bytes + off;
 }
 
 __device__ inline void* user_ptr() { return g_ctx.user; }
@@ -227,7 +243,7 @@ __device__ uint64_t reserve(int dst, uint64_t slot) {
     uint64_t spins = 0;
     flush_publish();  // never wait on another PE with messages held back
     while (ld_relaxed(&c.heads[(uint64_t)dst * 16]) < need) {
-      if (dst == (int)blockIdx.x) {
+      if (dst == pe_index()) {
         raise_error(ERR_SELF_RING_FULL, (uint32_t)slot);
         return kFail;
       }
@@ -315,7 +331,7 @@ __device__ void send_msg(int dst, msgtype type, int chare_id, int chare_idx,
 // deferral store (refnum mismatches), owner-only
 // ---------------------------------------------------------------------------
 __device__ inline char* arena() {
-  return g_ctx.deferred + (uint64_t)blockIdx.x * g_ctx.deferred_bytes;
+  return g_ctx.deferred + (uint64_t)pe_index() * g_ctx.deferred_bytes;
 }
 
 // Move [src, src+bytes) down to dst < src, in workgroup-wide chunks whose
@@ -460,8 +476,8 @@ __device__ int poll_next() {
   PeState* ps = pe_state();
   Lds& L = lds();
   const uint64_t NG = n_granules();
-  const uint64_t* rdy = c.ready + (uint64_t)blockIdx.x * NG;
-  char* ring = c.rings + (uint64_t)blockIdx.x * c.ring_stride;
+  const uint64_t* rdy = c.ready + (uint64_t)pe_index() * NG;
+  char* ring = c.rings + (uint64_t)pe_index() * c.ring_stride;
   for (uint32_t spins = 0;; spins++) {
     if (ps->n_revived > 0) {
       int best = -1;
@@ -533,7 +549,7 @@ __device__ __noinline__ void process_inbox() {
     flush_publish();
     wave_drain();
     ps->read_pos += slot;
-    st_relaxed(&c.heads[(uint64_t)blockIdx.x * 16], ps->read_pos);
+    st_relaxed(&c.heads[(uint64_t)pe_index() * 16], ps->read_pos);
     if (ok) ps->stats.delivered++;
     ps->stats.overhead_ticks += wall_clock64() - lds().t_found;
   }
@@ -599,8 +615,10 @@ __device__ __noinline__ void pe_loop() {
   }
 }
 
-// Collective grid barrier over all PEs (generation counter).  Replaces
-// nvshmem_barrier_all + grid.sync (scheduler.cu:336-350).
+// Collective barrier over every workgroup of the launch (generation
+// counter): all PEs in the one-PE-per-workgroup mode, the one PE in
+// GPU-level mode.  Replaces nvshmem_barrier_all + grid.sync
+// (scheduler.cu:336-350).
 __device__ void grid_barrier() {
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -611,7 +629,7 @@ __device__ void grid_barrier() {
     release_fence();
     const uint32_t arrived = __hip_atomic_fetch_add(
         cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (arrived == (uint32_t)g_ctx.n_pes - 1) {
+    if (arrived == gridDim.x - 1) {
       fence_acquire();
       __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       st_release(gen, g + 1);
@@ -632,11 +650,18 @@ __device__ void grid_barrier() {
 // Thread 0 at kernel start: empty the inbox (skip anything a previous run
 // left in flight), reset owner state.
 __device__ void pe_begin() {
+#ifdef CHARMING_GPU_LEVEL
+  // every workgroup empties its own publication list; the leader's
+  // workgroup resets the one PE's state
+  if (threadIdx.x == 0) lds().n_pend = 0;
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+#else
   if (threadIdx.x == 0) {
+#endif
     PeState* ps = pe_state();
-    const uint64_t t = ld_relaxed(&g_ctx.tails[(uint64_t)blockIdx.x * 16]);
+    const uint64_t t = ld_relaxed(&g_ctx.tails[(uint64_t)pe_index() * 16]);
     ps->read_pos = t;
-    st_relaxed(&g_ctx.heads[(uint64_t)blockIdx.x * 16], t);
+    st_relaxed(&g_ctx.heads[(uint64_t)pe_index() * 16], t);
     ps->table.count = 0;
     ps->heap_used = 0;
     ps->arena_used = 0;
@@ -650,15 +675,301 @@ __device__ void pe_begin() {
     lds().n_pend = 0;
     for (int i = 0; i < CHARMING_DEFERRED_MAX; i++) ps->deferred[i].state = 0;
     ps->stats = Stats{};
+    ps->n_user_free = g_ctx.user_bytes ? 1 : 0;
+    ps->user_free[0].offset = 0;
+    ps->user_free[0].size = g_ctx.user_bytes;
   }
   __syncthreads();
 }
+
+#ifdef CHARMING_GPU_LEVEL
+// ---------------------------------------------------------------------------
+// GPU-level mode: one PE spanning every workgroup
+// ---------------------------------------------------------------------------
+// words of g_ctx.bcast, written by the leader before a grid barrier and read
+// by every workgroup after it
+enum : int { BC_CMD = 0, BC_PTR = 1, BC_PROXY = 2, BC_IDX = 3, BC_EP = 4,
+             BC_POS = 5, BC_BASE = 6, BC_DST = 7, BC_BUF = 8, BC_SIZE = 9 };
+enum : int { CMD_RUN = 16, CMD_SKIP = 17 };
+
+__device__ inline bool is_leader() {
+  return blockIdx.x == 0 && threadIdx.x == 0;
+}
+__device__ inline uint64_t bc_get(int k) { return ld_relaxed(&g_ctx.bcast[k]); }
+__device__ inline void bc_put(int k, uint64_t v) { st_relaxed(&g_ctx.bcast[k], v); }
+
+// Copy by every thread of the launch: 16-byte words when the source allows
+// it, bytes otherwise.  dst is 16-byte aligned.
+__device__ void grid_copy(char* dst, const char* src, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t T = (size_t)gridDim.x * blockDim.x;
+  if (((uintptr_t)src & 15) == 0) {
+    const size_t n16 = n >> 4;
+    for (size_t i = t; i < n16; i += T)
+      ((uint4*)dst)[i] = ((const uint4*)src)[i];
+    for (size_t k = (n16 << 4) + t; k < n; k += T) dst[k] = src[k];
+  } else {
+    for (size_t k = t; k < n; k += T) dst[k] = src[k];
+  }
+}
+
+// Leader: reserve a message of `payload` bytes in dst's ring and write its
+// headers; returns the ring position (kFail on failure) and the address.
+__device__ uint64_t leader_reserve(int dst, msgtype type, int chare_id,
+                                   int chare_idx, int ep, int refnum,
+                                   size_t payload, char** base_out) {
+  const Ctx& c = g_ctx;
+  const uint64_t alloc = envelope::alloc_size(type, payload);
+  const uint64_t pos = reserve(dst, (alloc + kGranule - 1) & ~(kGranule - 1));
+  *base_out = nullptr;
+  if (pos == kFail) return pos;
+  char* base = c.rings + (uint64_t)dst * c.ring_stride + (pos & (c.ring_bytes - 1));
+  new (base) envelope(type, alloc);
+  new (base + sizeof(envelope)) regular_msg(chare_id, chare_idx, ep, refnum);
+  *base_out = base;
+  return pos;
+}
+
+// Grid-collective send (get.cu:1250-1275): barrier, the leader reserves
+// and writes the headers, barrier, every workgroup copies part of the
+// payload from the leader's buffer, barrier, the leader queues the
+// publication.  The buffer must be readable from every workgroup (global
+// memory), as in the reference's GPU-level programs.
+__device__ void gpu_send_msg(int dst, msgtype type, int chare_id, int chare_idx,
+                             int ep, int refnum, const void* buf,
+                             size_t payload) {
+  grid_barrier();
+  if (is_leader()) {
+    char* base;
+    const uint64_t pos = leader_reserve(dst, type, chare_id, chare_idx, ep,
+                                        refnum, payload, &base);
+    bc_put(BC_POS, pos);
+    bc_put(BC_BASE, (uint64_t)base);
+    bc_put(BC_DST, (uint64_t)dst);
+    bc_put(BC_BUF, (uint64_t)buf);
+    bc_put(BC_SIZE, payload);
+  }
+  grid_barrier();
+  const uint64_t pos = bc_get(BC_POS);
+  if (pos == kFail) return;
+  char* base = (char*)bc_get(BC_BASE);
+  const uint64_t n = bc_get(BC_SIZE);
+  const int to = (int)bc_get(BC_DST);
+  if (n)
+    grid_copy(base + sizeof(envelope) + sizeof(regular_msg),
+              (const char*)bc_get(BC_BUF), n);
+  wave_drain();
+  grid_barrier();
+  if (is_leader()) queue_publish(to, pos);
+}
+
+// Workgroup 0, collectively: poll the inbox, decide the refnum gate, defer a
+// mismatched message, and broadcast what every workgroup runs.
+__device__ __noinline__ void gpu_next() {
+  Lds& L = lds();
+  PeState* ps = pe_state();
+  if (threadIdx.x == 0) {
+    L.g_kind = poll_next();
+    L.g_run = 0;
+    L.g_slot = 0;
+  }
+  __syncthreads();
+  const int kind = L.g_kind;
+  int proxy = -1, idx = -1, ep = -1, defer_it = 0;
+  void* arg = nullptr;
+  if (threadIdx.x == 0 && kind != CMD_EXIT) {
+    ProxyTable& t = ps->table;
+    if (kind == CMD_ASYNC) {
+      proxy = L.arg;
+      idx = L.n;
+      ep = (int)L.pos;
+      if (proxy >= 0 && proxy < t.count && t.proxies[proxy] &&
+          t.proxies[proxy]->gate(idx, ep, -1) > 0)
+        L.g_run = 1;
+    } else {
+      envelope* env = (envelope*)L.ptr;
+      const uint64_t alloc = env->size;
+      if (kind == CMD_INBOX) L.g_slot = (alloc + kGranule - 1) & ~(kGranule - 1);
+      regular_msg* m = (regular_msg*)((char*)env + sizeof(envelope));
+      if (alloc < sizeof(envelope) + sizeof(regular_msg) ||
+          alloc > g_ctx.max_msg_bytes ||
+          (env->type != msgtype::regular && env->type != msgtype::user) ||
+          m->chare_id < 0 || m->chare_id >= t.count) {
+        raise_error(ERR_BAD_MESSAGE, (uint32_t)alloc);
+      } else {
+        proxy = m->chare_id;
+        idx = m->chare_idx;
+        ep = m->ep_id;
+        arg = (char*)m + sizeof(regular_msg);
+        const int g = t.proxies[proxy]->gate(idx, ep, m->refnum);
+        L.g_run = g > 0;
+        defer_it = (g == 0 && kind == CMD_INBOX);
+      }
+    }
+    L.n = defer_it;
+    bc_put(BC_CMD, L.g_run ? CMD_RUN : CMD_SKIP);
+    bc_put(BC_PROXY, (uint64_t)proxy);
+    bc_put(BC_IDX, (uint64_t)idx);
+    bc_put(BC_EP, (uint64_t)ep);
+    bc_put(BC_PTR, (uint64_t)arg);
+  }
+  if (threadIdx.x == 0 && kind == CMD_EXIT) bc_put(BC_CMD, CMD_EXIT);
+  __syncthreads();
+  if (kind == CMD_INBOX && L.n) {
+    envelope* env = (envelope*)L.ptr;
+    regular_msg* m = (regular_msg*)((char*)env + sizeof(envelope));
+    defer((const char*)env, env->size, m->chare_id, m->chare_idx, m->refnum);
+  }
+}
+
+// Leader, after every workgroup finished the entry method: publish its
+// sends, release the inbox space or settle the revived entry.
+__device__ void gpu_finish() {
+  Lds& L = lds();
+  PeState* ps = pe_state();
+  flush_publish();
+  if (L.g_kind == CMD_INBOX) {
+    wave_drain();
+    ps->read_pos += L.g_slot;
+    st_relaxed(&g_ctx.heads[(uint64_t)pe_index() * 16], ps->read_pos);
+  } else if (L.g_kind == CMD_REVIVED) {
+    DeferredEntry& d = ps->deferred[L.arg];
+    if (L.g_run) {
+      d.state = 0;
+      ps->n_deferred--;
+      ps->stats.revived++;
+      if (ps->n_deferred == 0) {
+        ps->arena_used = 0;
+        ps->hw = 0;
+      }
+    } else {
+      d.state = 1;  // refnum moved on again: stay deferred
+    }
+  }
+  if (L.g_run) ps->stats.delivered++;
+}
+
+__device__ __noinline__ void gpu_pe_loop() {
+  for (;;) {
+    if (blockIdx.x == 0) gpu_next();
+    grid_barrier();
+    const int cmd = (int)bc_get(BC_CMD);
+    if (cmd == CMD_EXIT) break;
+    if (cmd == CMD_RUN) {
+      ProxyTable& t = pe_state()->table;
+      t.proxies[(int)bc_get(BC_PROXY)]->run((int)bc_get(BC_IDX),
+                                           (int)bc_get(BC_EP),
+                                           (void*)bc_get(BC_PTR));
+    }
+    grid_barrier();
+    if (is_leader()) gpu_finish();
+  }
+}
+#endif  // CHARMING_GPU_LEVEL
 
 }  // namespace rt
 
 // ---------------------------------------------------------------------------
 // charm:: runtime functions (charming.h, message.h)
 // ---------------------------------------------------------------------------
+#ifdef CHARMING_GPU_LEVEL
+// GPU-level: grid-collective (every thread of every workgroup), the
+// leader's arguments count.
+__device__ void send_reg_msg(int chare_id, int chare_idx, int ep_id, void* buf,
+                             size_t payload_size, int dst_pe, int refnum) {
+  rt::gpu_send_msg(dst_pe, msgtype::regular, chare_id, chare_idx, ep_id,
+                   refnum, buf, payload_size);
+}
+
+__device__ void* alloc_user_msg(message& msg, int chare_id, int chare_idx,
+                                int ep_id, size_t payload_size, int dst_pe,
+                                int refnum) {
+  rt::grid_barrier();
+  if (rt::is_leader()) {
+    char* base;
+    const uint64_t pos = rt::leader_reserve(dst_pe, msgtype::user, chare_id,
+                                            chare_idx, ep_id, refnum,
+                                            payload_size, &base);
+    rt::bc_put(rt::BC_POS, pos);
+    rt::bc_put(rt::BC_BASE, (uint64_t)base);
+    rt::bc_put(rt::BC_DST, (uint64_t)dst_pe);
+  }
+  rt::grid_barrier();
+  const uint64_t pos = rt::bc_get(rt::BC_POS);
+  msg.env = pos == rt::kFail ? nullptr : (envelope*)rt::bc_get(rt::BC_BASE);
+  msg.offset = pos;
+  msg.dst_pe = (int)rt::bc_get(rt::BC_DST);
+  return msg.payload();
+}
+
+__device__ void send_user_msg(message& msg) {
+  if (!msg.env) return;
+  rt::wave_drain();
+  rt::grid_barrier();
+  if (rt::is_leader()) rt::queue_publish(msg.dst_pe, msg.offset);
+  msg.env = nullptr;
+}
+
+// charming.h:31-34 (GPU-level only, as in the reference, get.cu:1347-1361).
+// Single-threaded.  Memory from the PE's user arena (chm_config.user_bytes),
+// first fit over a sorted list of free extents; free_user returns an extent
+// and merges it with its neighbours.  nullptr when nothing fits.
+__device__ void* malloc_user(size_t size, size_t& offset) {
+  rt::PeState* ps = rt::pe_state();
+  const uint64_t need = (size + 15) & ~15ull;
+  for (int i = 0; i < ps->n_user_free; i++) {
+    rt::UserExtent& e = ps->user_free[i];
+    if (e.size < need) continue;
+    offset = e.offset;
+    e.offset += need;
+    e.size -= need;
+    if (e.size == 0) {
+      for (int k = i; k + 1 < ps->n_user_free; k++)
+        ps->user_free[k] = ps->user_free[k + 1];
+      ps->n_user_free--;
+    }
+    return rt::g_ctx.user_arena + (uint64_t)rt::pe_index() * rt::g_ctx.user_bytes +
+           offset;
+  }
+  printf("[ERROR] PE %d: malloc_user of %llu bytes: user arena exhausted\n",
+         rt::pe_index(), (unsigned long long)size);
+  return nullptr;
+}
+
+__device__ void free_user(size_t size, size_t offset) {
+  rt::PeState* ps = rt::pe_state();
+  const uint64_t len = (size + 15) & ~15ull;
+  if (offset + len > rt::g_ctx.user_bytes) {
+    rt::raise_error(rt::ERR_BAD_CHARE, (uint32_t)(offset >> 4));
+    return;
+  }
+  int i = 0;
+  while (i < ps->n_user_free && ps->user_free[i].offset < offset) i++;
+  const bool left = i > 0 && ps->user_free[i - 1].offset +
+                                     ps->user_free[i - 1].size == offset;
+  const bool right = i < ps->n_user_free &&
+                     offset + len == ps->user_free[i].offset;
+  if (left && right) {
+    ps->user_free[i - 1].size += len + ps->user_free[i].size;
+    for (int k = i; k + 1 < ps->n_user_free; k++)
+      ps->user_free[k] = ps->user_free[k + 1];
+    ps->n_user_free--;
+  } else if (left) {
+    ps->user_free[i - 1].size += len;
+  } else if (right) {
+    ps->user_free[i].offset = offset;
+    ps->user_free[i].size += len;
+  } else if (ps->n_user_free < rt::kUserExtents) {
+    for (int k = ps->n_user_free; k > i; k--)
+      ps->user_free[k] = ps->user_free[k - 1];
+    ps->user_free[i].offset = offset;
+    ps->user_free[i].size = len;
+    ps->n_user_free++;
+  } else {
+    rt::raise_error(rt::ERR_HEAP_FULL, (uint32_t)rt::kUserExtents);
+  }
+}
+#else
 __device__ void send_reg_msg(int chare_id, int chare_idx, int ep_id, void* buf,
                              size_t payload_size, int dst_pe, int refnum) {
   rt::send_msg(dst_pe, msgtype::regular, chare_id, chare_idx, ep_id, refnum,
@@ -684,6 +995,7 @@ __device__ void send_user_msg(message& msg) {
   if (threadIdx.x == 0) rt::queue_publish(msg.dst_pe, msg.offset);
   msg.env = nullptr;
 }
+#endif
 
 // Single-threaded (get.cu:1316-1345): deferred messages of (chare, refnum)
 // become eligible again; the scheduler re-dispatches them before new inbox
@@ -716,11 +1028,11 @@ __device__ void end() {
 // charm::abort(): stops every PE and records ERR_USER_ABORT.  (The
 // reference stops only the calling PE, charming.cu:269-278, which leaves the
 // others waiting at the final barrier.)
-__device__ void abort() { rt::raise_error(rt::ERR_USER_ABORT, blockIdx.x); }
+__device__ void abort() { rt::raise_error(rt::ERR_USER_ABORT, rt::pe_index()); }
 
-__device__ int my_pe() { return (int)blockIdx.x; }
+__device__ int my_pe() { return rt::pe_index(); }
 __device__ int n_pes() { return rt::g_ctx.n_pes; }
-__device__ int my_pe_node() { return (int)blockIdx.x; }
+__device__ int my_pe_node() { return rt::pe_index(); }
 __device__ int n_pes_node() { return rt::g_ctx.n_pes; }
 __device__ int n_nodes() { return 1; }
 
@@ -739,9 +1051,13 @@ __device__ int device_atoi(const char* str, int strlen) {
 }
 
 __device__ void flush_sends() {
+#ifdef CHARMING_GPU_LEVEL
+  rt::grid_barrier();  // the leader's pending publications are flushed there
+#else
   __syncthreads();
   if (threadIdx.x == 0) rt::flush_publish();
   __syncthreads();
+#endif
 }
 
 __device__ double wall_time() {
@@ -761,7 +1077,7 @@ namespace rt {
 // an entry-method pointer is compiled for up to 1024-thread workgroups
 // (128 VGPRs), and the kernel must not add registers of its own on top.
 __device__ __noinline__ void run_user_main() {
-  charm::main(g_ctx.argc, g_ctx.argv, g_ctx.argvs, (int)blockIdx.x);
+  charm::main(g_ctx.argc, g_ctx.argv, g_ctx.argvs, pe_index());
   __syncthreads();
   if (threadIdx.x == 0) flush_publish();
 }
@@ -771,7 +1087,11 @@ __device__ __noinline__ void pe_run() {
   grid_barrier();
   run_user_main();
   grid_barrier();
+#ifdef CHARMING_GPU_LEVEL
+  gpu_pe_loop();
+#else
   pe_loop();
+#endif
   grid_barrier();
 }
 
@@ -795,7 +1115,13 @@ struct ProgramRegistrar {
     p.threads_per_pe = CHARMING_PE_THREADS;
     p.main_host = &charm::main_host;
     p.set_ctx = &set_ctx_host;
+#ifdef CHARMING_GPU_LEVEL
+    p.name = __BASE_FILE__ " (gpu-level)";
+    p.gpu_level = 1;
+#else
     p.name = __BASE_FILE__;  // the including program, not this header
+    p.gpu_level = 0;
+#endif
     chm_register_program(&p);
   }
 };

@@ -18,9 +18,17 @@
   } while (0)
 #endif
 
-#ifndef SM_LEVEL
-// This runtime always maps one PE to one workgroup (the reference's
-// SM_LEVEL mode, charming.cu:85-92); examples select that branch.
+#ifdef CHARMING_GPU_LEVEL
+// GPU-level mode (the reference's build without SM_LEVEL): one PE spans
+// every workgroup of the launch.  GID and BID as in the reference's
+// common.h:90-91.
+#undef SM_LEVEL
+#define GID (blockDim.x * blockIdx.x + threadIdx.x)
+#define BID (blockIdx.x)
+#elif !defined(SM_LEVEL)
+// Default: one PE per workgroup (the reference's SM_LEVEL mode,
+// charming.cu:85-92); programs select that branch.  Define
+// CHARMING_GPU_LEVEL before including <charming.h> for GPU-level mode.
 #define SM_LEVEL 1
 #endif
 

@@ -10,10 +10,12 @@ pytestmark = pytest.mark.skipif(not R.available(),
                                 reason="reference examples not present")
 
 
-@pytest.mark.parametrize("name,subs", [("hello", 0), ("refnum", 1),
-                                       ("pingpong", 15)])
-def test_reference_example_compiles_unchanged(name, subs):
-    rc, log, n_sub = R.compile_example(name)
+@pytest.mark.parametrize("name,subs,defs", [
+    ("hello", 0, ()), ("refnum", 1, ()), ("pingpong", 15, ()),
+    # the reference's default build (no SM_LEVEL): its GPU-level branches
+    ("pingpong", 15, ("CHARMING_GPU_LEVEL",))])
+def test_reference_example_compiles_unchanged(name, subs, defs):
+    rc, log, n_sub = R.compile_example(name, defs)
     assert rc == 0, log[-3000:]
     # the only edits: the chrono include and the cuda::std::chrono spelling
     assert n_sub == subs
