@@ -18,13 +18,20 @@
 //     is harmless only for square chare grids);
 //   - the last row/column of chares takes the remainder of an uneven split;
 //   - a chare with no neighbours iterates without waiting;
-//   - temporal blocking (halo depth D = 2 .. 8): chares
+//   - temporal blocking (halo depth D = 2 .. 16): chares
 //     exchange halos D cells deep (four edges of D rows/columns plus four
 //     (D-1) x (D-1) corner blocks from the diagonal neighbours) once per D
 //     sweeps, and sweep D times in one pass over their block with the
 //     intermediate grids held in registers; the arithmetic, and so every
-//     grid, is unchanged.
-#define CHARMING_PE_THREADS 1024
+//     grid, is unchanged;
+//   - the sweep rounds run in the runtime's work hook (requested by the
+//     halo entry method once every halo is in), which the scheduler calls
+//     directly: compiled for the 512-thread PE (256 VGPRs per lane) rather
+//     than an entry method's 128, so sixteen fused sweeps fit in registers
+//     and no callee-saved registers are spilled around an indirect call.
+#define CHARMING_PE_THREADS 512
+__device__ void jacobi_work(void* token);
+#define CHARMING_WORK_HOOK jacobi_work
 #include <charming.h>
 
 #include <stdlib.h>
@@ -68,9 +75,10 @@ struct JacobiDesc {
                           // sweep writes them into the messages in place)
 };
 
-// Chare arrays always hold halos of the deepest supported depth, eight: rows
-// -7 .. bh+8 of pitch_for_depth(bw, 8) doubles; grids[] point at row 0.
-constexpr int kHaloRows = 8;
+// Chare arrays always hold halos of the deepest supported depth, sixteen:
+// rows -15 .. bh+16 of pitch_for_depth(bw, 16) doubles; grids[] point at
+// row 0.
+constexpr int kHaloRows = 16;
 __host__ __device__ inline int64_t chare_pitch(int bw) {
   return pitch_for_depth(bw, kHaloRows);
 }
@@ -132,6 +140,11 @@ struct Block : charm::chare {
 __device__ void entry_init(Block& b, void* arg) { b.init(arg); }
 __device__ void entry_recv_ghosts(Block& b, void* arg) { b.recv_ghosts(arg); }
 __device__ void entry_end(Block& b, void* arg) { b.end(arg); }
+
+// The work hook: the sweep rounds of a chare whose halos are all in.  Only
+// reached from the scheduler's direct call (never through an entry method),
+// so everything below it is compiled for the PE's 512 threads.
+__device__ void jacobi_work(void* token) { ((Block*)token)->update(); }
 
 __shared__ charm::chare_proxy<Block>* block_proxy;
 
@@ -310,7 +323,7 @@ __device__ inline T* wuni(T* p) {
 template <int D, bool RES>
 __device__ __noinline__ void chare_sweepD(const double* in, double* out,
                                           int64_t P, int bw, int bh,
-                                          RingSides rs) {
+                                          RingSides rs, int n_copy) {
   in = wuni(in);
   out = wuni(out);
   P = wuni(P);
@@ -320,6 +333,7 @@ __device__ __noinline__ void chare_sweepD(const double* in, double* out,
   rs.bottom = wuni((int)rs.bottom);
   rs.left = wuni((int)rs.left);
   rs.right = wuni((int)rs.right);
+  n_copy = wuni(n_copy);
   // (the wave index stays a vector value: moved to a scalar register, the
   // row bookkeeping derived from it raises scalar-register pressure enough
   // to spill in the deeper sweeps, measured slower)
@@ -328,10 +342,12 @@ __device__ __noinline__ void chare_sweepD(const double* in, double* out,
   const int S = (bw + OW - 1) / OW;
   u128acc acc = {0, 0};
   const EdgeOut2 none = {nullptr, nullptr, nullptr, nullptr};
-  // bands of 256 rows for tall chares (measured best on 1024-row chares);
-  // shorter ones are cut into contiguous shares
+  // bands of 256 rows for tall chares at up to eight fused sweeps
+  // (measured best on 1024-row chares); shorter chares, and deeper passes
+  // (whose pipeline fill of 2D-2 rows a short band repeats), are cut into
+  // contiguous shares
   int band = wuni(g_jdesc.tune_band);
-  if (band == 0) band = bh >= 1024 ? 256 : -1;
+  if (band == 0) band = (bh >= 1024 && D <= 8) ? 256 : -1;
   if (band > 0) {
     // bands of `band` rows, windows of a band side by side: waves take the
     // tiles round-robin, so neighbouring windows (which read overlapping
@@ -341,7 +357,7 @@ __device__ __noinline__ void chare_sweepD(const double* in, double* out,
       const int panel = t % S, b = t / S;
       const int r0 = 1 + b * band, r1 = min(r0 + band, bh + 1);
       sweep_panelD<D, RES, false>(in, out, P, bw, bh, 1 + panel * OW, r0, r1,
-                                  acc, none, rs);
+                                  acc, none, rs, -1, n_copy);
     }
   } else {
     // the S windows x bh rows, window after window, cut into one contiguous
@@ -354,7 +370,7 @@ __device__ __noinline__ void chare_sweepD(const double* in, double* out,
       const int panel = (int)(pos / bh), r = (int)(pos % bh);
       const int len = (int)min((int64_t)(bh - r), end - pos);
       sweep_panelD<D, RES, false>(in, out, P, bw, bh, 1 + panel * OW, 1 + r,
-                                  1 + r + len, acc, none, rs);
+                                  1 + r + len, acc, none, rs, -1, n_copy);
       pos += len;
     }
   }
@@ -364,21 +380,40 @@ __device__ __noinline__ void chare_sweepD(const double* in, double* out,
   }
 }
 
-// `steps` (3 .. kHaloRows) fused sweeps in one pass.
+// `steps` (3 .. kHaloRows) fused sweeps in one pass: 3 .. 8 by their own
+// pass (112-column windows), 9 .. 16 by the sixteen-deep pass (96-column
+// windows) whose first 16 - steps stages only copy.
 __device__ void sweepD_dispatch(int steps, const double* in, double* out,
                                 int64_t P, int bw, int bh, bool res,
                                 RingSides rs) {
 #define CHM_CASED(D_)                                            \
   case D_:                                                       \
     if (res)                                                     \
-      chare_sweepD<D_, true>(in, out, P, bw, bh, rs);            \
+      chare_sweepD<D_, true>(in, out, P, bw, bh, rs, 0);         \
     else                                                         \
-      chare_sweepD<D_, false>(in, out, P, bw, bh, rs);           \
+      chare_sweepD<D_, false>(in, out, P, bw, bh, rs, 0);        \
     break;
   switch (steps) {
     CHM_CASED(3) CHM_CASED(4) CHM_CASED(5) CHM_CASED(6) CHM_CASED(7)
     CHM_CASED(8)
-    default: break;
+    default:
+      if (steps > 8 && steps <= 12) {
+        if (res)
+          chare_sweepD<12, true>(in, out, P, bw, bh, rs, 12 - steps);
+        else
+          chare_sweepD<12, false>(in, out, P, bw, bh, rs, 12 - steps);
+      } else if (steps > 12 && steps <= 14) {
+        if (res)
+          chare_sweepD<14, true>(in, out, P, bw, bh, rs, 14 - steps);
+        else
+          chare_sweepD<14, false>(in, out, P, bw, bh, rs, 14 - steps);
+      } else if (steps > 14 && steps <= 16) {
+        if (res)
+          chare_sweepD<16, true>(in, out, P, bw, bh, rs, 16 - steps);
+        else
+          chare_sweepD<16, false>(in, out, P, bw, bh, rs, 16 - steps);
+      }
+      break;
   }
 #undef CHM_CASED
 }
@@ -510,7 +545,7 @@ __device__ void Block::init(void* arg) {
   __syncthreads();
   send_boundaries();
   if (threadIdx.x == 0) t_mark = wall_clock64();
-  if (neighbor_count == 0) update();
+  if (neighbor_count == 0) charm::request_work(this);
 }
 
 // Bytes of the halo message toward neighbour k: the dir slot, then `depth`
@@ -679,7 +714,7 @@ __device__ void Block::recv_ghosts(void* arg) {
       g_jdesc.recv_ticks[charm::chare::i] += wall_clock64() - tr0;
   }
   __syncthreads();
-  if (go) update();
+  if (go) charm::request_work(this);
 }
 
 __device__ void Block::update() {

@@ -565,10 +565,11 @@ __device__ inline void store_pair(gdouble* orow, int c, int cend, int w, int h,
 // Deep fusion works on overlapping windows: a wave holds 128 columns of a
 // row (lane l: the pair at w0+2l, w0+2l+1), each sweep leaves one more
 // column at either end without a valid neighbour, and after D <= 8 sweeps
-// the middle 112 columns are stored: margins of H = 8 columns whatever D
-// is, so a window's output is seven whole 128-byte lines (column 1 starts a
-// line) and every store is line-aligned.  Neighbouring windows overlap by
-// 2H columns; the lane shifts need no boundary fix-ups.
+// the middle 112 columns are stored (margins of H = 8 columns whatever D
+// is), after 9 <= D <= 16 sweeps the middle 96 (H = 16): a window's output
+// is seven or six whole 128-byte lines (column 1 starts a line) and every
+// store is line-aligned.  Neighbouring windows overlap by 2H columns; the
+// lane shifts need no boundary fix-ups.
 struct RowW {
   double2 v;
 };
@@ -587,9 +588,11 @@ __device__ inline double lane_shl1z(double v) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// Columns of a window's margin, and its output columns, for D <= 8 fused
+// Columns of a window's margin, and its output columns, for D <= 16 fused
 // sweeps.
-__host__ __device__ constexpr int window_margin(int D) { return D <= 8 ? 8 : -1; }
+__host__ __device__ constexpr int window_margin(int D) {
+  return D <= 8 ? 8 : (D <= 16 ? 16 : -1);
+}
 __host__ __device__ constexpr int window_out_cols(int D) {
   return 128 - 2 * window_margin(D);
 }
@@ -721,7 +724,7 @@ __device__ inline void sweep_panelD(const double* __restrict__ in,
                                     double* accd = nullptr) {
   static_assert(RALL != 1 || RES, "per-sweep residuals include the last one");
   static_assert(RALL != 2 || !RES, "the FP64 sums replace the exact ones");
-  static_assert(D >= 3 && D <= 8, "deep fused sweep: 3 <= D <= 8");
+  static_assert(D >= 3 && D <= 16, "deep fused sweep: 3 <= D <= 16");
   if (r0 >= r1) return;
   PanelW p;
   p.setup(in, out, pitch, w, h, c0, cend < 0 ? w : cend, rs, D);

@@ -53,6 +53,13 @@ __device__ double wall_time();
 // sends are published one by one.)
 __device__ void flush_sends();
 
+// Ask the scheduler to call the program's work hook (CHARMING_WORK_HOOK,
+// defined before including this header) with `token` once the current entry
+// method returns.  The hook is called directly, so the compiler gives it the
+// scheduler kernel's register budget instead of an entry method's.  (No
+// counterpart in the reference.)
+__device__ void request_work(void* token);
+
 }  // namespace charm
 
 #ifndef CHARMING_NO_RUNTIME

@@ -66,6 +66,8 @@ struct Lds {
   uint64_t g_slot;   // GPU-level leader: ring bytes of the current message
   int g_kind;        // GPU-level leader: what poll_next found (CMD_*)
   int g_run;         // GPU-level leader: the entry method runs (1) or not
+  void* work;        // token of the program's work hook (request_work)
+  int has_work;
   uint64_t t_found;  // wall clock when the current message was found
   int cmd;
   int arg;
@@ -598,6 +600,33 @@ __device__ __noinline__ void process_async() {
   }
 }
 
+// The program's work hook: a chare program that defines CHARMING_WORK_HOOK
+// (the name of a `__device__ void f(void* token)`) before including
+// <charming.h> may ask from an entry method for that function to run once
+// the entry method has returned (charm::request_work(token)).  The scheduler
+// calls it directly, not through a function pointer, so the compiler sizes
+// it for the scheduler kernel's workgroup (up to 256 VGPRs per lane for
+// 512-thread PEs, and a static stack) instead of the 128 VGPRs and dynamic
+// stack of code reached through an entry-method pointer.  It runs
+// workgroup-collectively like an entry method and may send; its sends are
+// published when it returns.  (No counterpart in the reference.)
+__device__ __noinline__ void run_work() {
+#ifdef CHARMING_WORK_HOOK
+  Lds& L = lds();
+  for (;;) {
+    __syncthreads();
+    if (!L.has_work) break;
+    void* tok = L.work;
+    __syncthreads();
+    if (threadIdx.x == 0) L.has_work = 0;
+    __syncthreads();
+    CHARMING_WORK_HOOK(tok);  // may request more work
+    __syncthreads();
+    if (threadIdx.x == 0) flush_publish();
+  }
+#endif
+}
+
 __device__ __noinline__ void pe_loop() {
   Lds& L = lds();
   for (;;) {
@@ -611,6 +640,7 @@ __device__ __noinline__ void pe_loop() {
       process_revived();
     else
       process_async();
+    run_work();
     __syncthreads();
   }
 }
@@ -653,7 +683,10 @@ __device__ void pe_begin() {
 #ifdef CHARMING_GPU_LEVEL
   // every workgroup empties its own publication list; the leader's
   // workgroup resets the one PE's state
-  if (threadIdx.x == 0) lds().n_pend = 0;
+  if (threadIdx.x == 0) {
+    lds().n_pend = 0;
+    lds().has_work = 0;
+  }
   if (threadIdx.x == 0 && blockIdx.x == 0) {
 #else
   if (threadIdx.x == 0) {
@@ -673,6 +706,7 @@ __device__ void pe_begin() {
     ps->n_waits = 0;
     for (int i = 0; i < ASYNC_WAIT_MAX; i++) ps->waits[i].valid = 0;
     lds().n_pend = 0;
+    lds().has_work = 0;
     for (int i = 0; i < CHARMING_DEFERRED_MAX; i++) ps->deferred[i].state = 0;
     ps->stats = Stats{};
     ps->n_user_free = g_ctx.user_bytes ? 1 : 0;
@@ -1060,6 +1094,16 @@ __device__ void flush_sends() {
 #endif
 }
 
+// Ask for the program's work hook to run with `token` once the current
+// entry method returns (any thread; the last request before the return
+// wins).  Workgroup-level mode.
+__device__ void request_work(void* token) {
+  if (threadIdx.x == 0) {
+    rt::lds().work = token;
+    rt::lds().has_work = 1;
+  }
+}
+
 __device__ double wall_time() {
   return (double)wall_clock64() / rt::g_ctx.wall_hz;
 }
@@ -1080,6 +1124,9 @@ __device__ __noinline__ void run_user_main() {
   charm::main(g_ctx.argc, g_ctx.argv, g_ctx.argvs, pe_index());
   __syncthreads();
   if (threadIdx.x == 0) flush_publish();
+#ifndef CHARMING_GPU_LEVEL
+  run_work();
+#endif
 }
 
 __device__ __noinline__ void pe_run() {

@@ -58,8 +58,10 @@ def edited_copy(name, dst_dir):
         for old, new in EDITS:
             n_sub += text.count(old)
             text = text.replace(old, new)
-        # hipcc compiles HIP from a .hip name; the text is unchanged
-        out_name = f[:-3] + ".hip" if f.endswith(".cu") else f
+        # hipcc compiles HIP from a .hip name; the text is unchanged.  The
+        # program registers under this file name: "<name>_reference.hip"
+        # keeps it apart from this repository's own <name>.hip
+        out_name = f[:-3] + "_reference.hip" if f.endswith(".cu") else f
         with open(os.path.join(dst_dir, out_name), "w") as fh:
             fh.write(text)
         if f == name + ".cu":
@@ -70,8 +72,8 @@ def edited_copy(name, dst_dir):
 def compile_example(name, defines=(), link=False, out=None):
     """Compiles (and with link=True links) one reference example: the
     executable `out` (with the runtime's main) and the loadable library
-    lib<out>.so beside it, which registers the program under a source path
-    containing "chm_ref_<name>_" (for chm_select_program).  Returns
+    lib<out>.so beside it, which registers the program under the source path
+    ".../chm_ref_<name>_*/<name>_reference.hip" (for chm_select_program).  Returns
     (returncode, compiler output, substitutions made)."""
     with tempfile.TemporaryDirectory(prefix=f"chm_ref_{name}_") as d:
         cu, n_sub = edited_copy(name, d)

@@ -33,7 +33,7 @@ def _run(capfd, name, timeout_s, n_pes=None):
     rt = _loaded["rt"]
     if name not in _loaded:
         _loaded[name] = ctypes.CDLL(path)
-    assert rt.chm_select_program(f"chm_ref_{name}_".encode()) == 0
+    assert rt.chm_select_program(f"{name}_reference.hip".encode()) == 0
     os.environ["CHM_TIMEOUT_S"] = str(timeout_s)
     os.environ["CHM_STATS"] = "1"
     if n_pes:
