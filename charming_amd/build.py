@@ -89,9 +89,18 @@ def _compile(src, verbose):
     return src
 
 
+# Shared libraries bind their own symbols (-Bsymbolic).  Every chare
+# program instantiates the scheduler kernel template, and programs with the
+# same workgroup size instantiate the same name; without it, a library
+# loaded after another one that exported that name with RTLD_GLOBAL (as
+# ctypes loads libchm_jacobi2d.so) would bind its kernel handle to the other
+# library's and launch the other program's scheduler.
+SHARED_LINK_FLAGS = ["-shared", "-fPIC", "-Wl,-Bsymbolic"]
+
+
 def _link(name, srcs, deps, verbose):
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o",
-           os.path.join(LIBDIR, name)] + [_obj(s) for s in srcs]
+    cmd = [HIPCC, "--offload-arch=gfx950"] + SHARED_LINK_FLAGS + [
+        "-o", os.path.join(LIBDIR, name)] + [_obj(s) for s in srcs]
     cmd += ["-L", LIBDIR] + [f"-l{d}" for d in deps]
     cmd += ["-Wl,-rpath,$ORIGIN", "-lpthread"]
     if verbose:

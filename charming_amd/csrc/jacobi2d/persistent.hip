@@ -70,7 +70,7 @@ struct JacobiDesc {
                           // chare height
   int overlap;            // sweep edges first, send, then the interior
   uint64_t* recv_ticks;   // per chare: ticks inside recv_ghosts (unpack)
-  int depth;              // halo depth = sweeps per exchange (1 .. 8)
+  int depth;              // halo depth = sweeps per exchange (1 .. 16)
   int halo_copy;          // 1: halos staged and copied by invoke (else the
                           // sweep writes them into the messages in place)
 };

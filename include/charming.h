@@ -13,8 +13,11 @@
 namespace charm {
 
 // Main function executed on the host before the launch (charming.cu:73).
-// Optional: a program that does not define it gets none.
-__attribute__((weak)) void main_host(int argc, char** argv);
+// Optional: a program that does not define it gets none.  Hidden, so that a
+// program library's reference resolves to its own definition (or to none)
+// and never to another program library's.
+__attribute__((weak, visibility("hidden"))) void main_host(int argc,
+                                                            char** argv);
 
 // Main function executed on all PEs (by every thread of each PE's workgroup,
 // or of every workgroup in GPU-level mode)

@@ -71,8 +71,8 @@ typedef struct chm_jacobi_tuning {
   int overlap;       /* persistent form: 1 = sweep the edges first and send the
                         halos before the interior (default 0; depth 1 only) */
   int depth;         /* persistent form: halo depth = sweeps per exchange,
-                        1 .. 8 (every chare at least depth x depth; 0 = as
-                        deep as the smallest chare allows, up to 8, with
+                        1 .. 16 (every chare at least depth x depth; 0 = as
+                        deep as the smallest chare allows, up to 16, with
                         the default or 1x1 sweep shape, else 2) */
   int halo_copy;     /* persistent form: 1 = pack halos into staging and send
                         them by copy (invoke); 0 = zero-copy: the sweep writes

@@ -100,7 +100,11 @@ def compile_example(name, defines=(), link=False, out=None):
             return p.returncode, log, n_sub
         so = os.path.join(os.path.dirname(exe),
                           "lib" + os.path.basename(exe) + ".so")
+        # -Bsymbolic: the library's scheduler kernel handle binds to its own
+        # definition, not to a same-named one exported by a library loaded
+        # before it (charming_amd.build.SHARED_LINK_FLAGS)
         p = subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC",
+                            "-Wl,-Bsymbolic",
                             "-o", so, obj, "-L", LIBDIR, "-lcharming_amd",
                             "-lpthread"] + rpath, capture_output=True, text=True)
         return p.returncode, log + p.stdout + p.stderr, n_sub

@@ -38,8 +38,9 @@ def test_c1_config_matches_chare_oracle(cam):
     assert r.residual == res
     assert r.hash == O.grid_hash(ref)
     # init + end messages, and per exchange round 48 edges and 36 corners
-    # (halos eight deep by default: one round per eight sweeps)
-    assert r.messages == expected_messages(4, 4, 100, 8)
+    # (halos as deep as the 128^2 chares allow, up to sixteen: one round per
+    # sixteen sweeps)
+    assert r.messages == expected_messages(4, 4, 100, 16)
 
 
 @pytest.mark.parametrize("W,H,cr,cc,iters", [
@@ -97,7 +98,7 @@ def test_default_decomposition_uses_all_pes(cam):
 
 
 @pytest.mark.parametrize("halo_copy", [0, 1])
-@pytest.mark.parametrize("depth", [1, 2, 3, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("depth", list(range(1, 17)))
 @pytest.mark.parametrize("W,H,cr,cc,iters", [
     (300, 200, 3, 5, 17), (257, 129, 4, 3, 10), (64, 64, 2, 2, 1),
     (100, 100, 5, 5, 2), (130, 70, 1, 1, 9), (500, 40, 2, 7, 5)])
@@ -134,7 +135,7 @@ def deep_cases(D):
         (512, 512, 4, 4, 100)]
 
 
-@pytest.mark.parametrize("depth", [3, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("depth", list(range(3, 17)))
 @pytest.mark.parametrize("case", range(5))
 def test_deep_halo_edge_cases(cam, depth, case):
     W, H, cr, cc, iters = deep_cases(depth)[case]
@@ -151,7 +152,11 @@ def test_deep_halo_edge_cases(cam, depth, case):
 @pytest.mark.parametrize("depth,iters,band", [(3, 12, 0), (4, 13, 0), (5, 14, 0),
                                               (6, 12, 0), (5, 11, 64), (6, 13, 7),
                                               (4, 9, 300), (7, 15, -1),
-                                              (8, 17, 256), (8, 16, 128)])
+                                              (8, 17, 256), (8, 16, 128),
+                                              (9, 20, 0), (11, 23, 64),
+                                              (12, 25, 0), (13, 29, -1),
+                                              (14, 30, 7), (15, 31, 0),
+                                              (16, 33, 0), (16, 48, 256)])
 def test_deep_fused_sweeps_reference_init(cam, depth, iters, band):
     W, H = 1000, 600
     ref, res = O.jacobi_run(W, H, iters, seed=23, mode=1)
@@ -173,7 +178,7 @@ def test_unsupported_fused_shape_is_refused(cam):
         j.set_tuning(1, 2, depth=4)
         with pytest.raises(cam.ChmError):
             j.solve()
-        j.set_tuning(depth=9)
+        j.set_tuning(depth=17)
         with pytest.raises(cam.ChmError):
             j.solve()
 
