@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "jacobi2d_internal.h"
 
@@ -292,6 +293,126 @@ static int check_problem(const chm_jacobi_problem* p) {
   return CHM_OK;
 }
 
+// ----------------------------------------------------------------------------
+// the no-alloc plain solve: its second grid and accumulators live in one
+// static device buffer
+// ----------------------------------------------------------------------------
+
+// Bytes of the static buffer; its first kNoAllocAccBytes hold the hash and
+// residual accumulators, the rest the second grid of the ping-pong.
+constexpr size_t kNoAllocScratchBytes = CHM_NOALLOC_SCRATCH_BYTES;
+constexpr size_t kNoAllocAccBytes = 256;
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / 8];
+
+// scratch's device address on each device (0 until looked up there)
+constexpr int kMaxDevices = 64;
+static void* g_scratch_addr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static int scratch_on_device(int dev, char** out) {
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(CHM_E_UNSUPPORTED, "device %d beyond the %d this build tracks",
+                dev, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(CHM_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s",
+                  dev, hipGetErrorString(e));
+    g_scratch_addr[dev] = ptr;
+  }
+  *out = (char*)g_scratch_addr[dev];
+  return CHM_OK;
+}
+
+// Clears n words (a kernel rather than hipMemsetAsync, so the call can be
+// captured into a graph).
+__global__ void zero_words_kernel(uint64_t* p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+// Copies the ring (rows 0 and H+1, columns 0 and W+1) of src to dst.
+__global__ void copy_ring_kernel(const double* src, double* dst, int W, int H,
+                                 int64_t P) {
+  const int64_t n = 2 * (int64_t)(W + 2) + 2 * (int64_t)H;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i, j;
+    if (e < 2 * (int64_t)(W + 2)) {
+      i = e < W + 2 ? 0 : H + 1;
+      j = e % (W + 2);
+    } else {
+      const int64_t k = e - 2 * (int64_t)(W + 2);
+      i = 1 + k / 2;
+      j = (k & 1) ? W + 1 : 0;
+    }
+    const int64_t o = i * P + kColOff + j;
+    dst[o] = src[o];
+  }
+}
+
+// Copies the interior (rows 1..H, columns 1..W) of src to dst.
+__global__ void copy_interior_kernel(const double* src, double* dst, int W,
+                                     int H, int64_t P) {
+  const int64_t n = (int64_t)W * H;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t o = (1 + e / W) * P + kColOff + 1 + e % W;
+    dst[o] = src[o];
+  }
+}
+
+__global__ void copy_words_kernel(const uint64_t* src, uint64_t* dst, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+
+static int solve_noalloc(double* grid, int W, int H, int iters,
+                         hipStream_t stream, uint64_t* out3) {
+  if (!grid || !out3) return fail(CHM_E_INVALID, "null grid or output");
+  if (W < 1 || H < 1 || iters < 0)
+    return fail(CHM_E_INVALID, "bad size %dx%d iters %d", W, H, iters);
+  if (((uintptr_t)grid & 127) != 0)
+    return fail(CHM_E_INVALID, "grid must be 128-byte aligned");
+  const int64_t P = pitch_for(W);
+  const size_t bytes = array_elems(W, H) * sizeof(double);
+  if ((int64_t)W > (1 << 30) || (int64_t)H > (1 << 30) ||
+      bytes > kNoAllocScratchBytes - kNoAllocAccBytes)
+    return fail(CHM_E_INVALID,
+                "a %dx%d grid needs %zu bytes of scratch; the no-alloc solve "
+                "holds at most %zu",
+                W, H, bytes, kNoAllocScratchBytes - kNoAllocAccBytes);
+  int dev = 0;
+  CHM_TRY_HIP(hipGetDevice(&dev));
+  char* base = nullptr;
+  CHM_TRY(scratch_on_device(dev, &base));
+  uint64_t* acc = (uint64_t*)base;
+  double* other = (double*)(base + kNoAllocAccBytes);
+  Handle tmp;
+  memset(&tmp, 0, sizeof(tmp));
+  tmp.W = W;
+  tmp.H = H;
+  StepLaunch L;
+  CHM_TRY(plan_step(&tmp, &L));
+  copy_ring_kernel<<<64, 256, 0, stream>>>(grid, other, W, H, P);
+  double* g[2] = {grid, other};
+  for (int it = 0; it < iters; it++)
+    hipLaunchKernelGGL(L.fn, L.grid, L.block, 0, stream, g[it & 1],
+                       g[(it + 1) & 1], W, H, P, L.band_rows, L.n_panels,
+                       L.n_tiles, L.n_blocks);
+  const int cur = iters & 1;
+  zero_words_kernel<<<1, 64, 0, stream>>>(acc, 4);
+  if (iters > 0)
+    residual_global_kernel<<<1024, 256, 0, stream>>>(g[cur], g[cur ^ 1], W, H,
+                                                     P, acc + 1);
+  hash_global_kernel<<<1024, 256, 0, stream>>>(g[cur], P, 1, H + 1, 1, W + 1,
+                                               acc);
+  if (cur) copy_interior_kernel<<<1024, 256, 0, stream>>>(other, grid, W, H, P);
+  copy_words_kernel<<<1, 64, 0, stream>>>(acc, out3, 3);
+  CHM_TRY_HIP(hipGetLastError());
+  return CHM_OK;
+}
+
 }  // namespace jac
 }  // namespace chm
 
@@ -500,6 +621,11 @@ int chm_hbm_copy_probe(size_t bytes, int reps, double* ms_per_copy) {
   (void)hipEventDestroy(e1);
   (void)hipStreamDestroy(s);
   return rc;
+}
+
+int chm_jacobi2d_solve_noalloc(double* grid, int width, int height,
+                               int iters, void* stream, uint64_t* out3) {
+  return solve_noalloc(grid, width, height, iters, (hipStream_t)stream, out3);
 }
 
 int chm_jacobi2d_destroy(void* handle) {

@@ -115,6 +115,31 @@ int chm_jacobi2d_phase_stats(void* handle, double* out8);
 
 int chm_jacobi2d_destroy(void* handle);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, serving loops and harnesses that
+ * forbid hipMalloc.  The plain per-launch solve of chm_jacobi2d_solve
+ * (CHM_FORM_PLAIN: one sweep launch per iteration, the ring held fixed) on a
+ * grid the caller owns:
+ *   grid   device memory, 128-byte aligned, in the library's layout:
+ *          height+2 rows of pitch = ((width + 18 + 15) / 16) * 16 doubles;
+ *          row i holds ring column 0, interior columns 1..width and ring
+ *          column width+1 at element offsets 15 .. 16+width.  u_0 on entry
+ *          (the ring is the fixed boundary), u_iters on return.
+ *   out3   device memory, 3 words: the order-independent hash of the
+ *          interior of u_iters, then the exact residual of the last sweep
+ *          (lo, hi), as in chm_jacobi_result.
+ *   stream a hipStream_t (NULL = the null stream).  The work is enqueued
+ *          and the call returns without waiting; it may be captured.
+ * The second grid of the ping-pong and the reduction accumulators live in
+ * one static device buffer of CHM_NOALLOC_SCRATCH_BYTES per GPU, held until
+ * the process exits; a grid whose (height+2) x pitch doubles exceed it less
+ * 256 bytes is refused with CHM_E_INVALID (up to about 2870 x 2870).  Every
+ * call on one GPU shares that buffer, so calls that overlap on one GPU (two
+ * streams, two threads) race.  Calls no hipMalloc, hipFree or hipMemset. */
+#define CHM_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int chm_jacobi2d_solve_noalloc(double* grid, int width, int height, int iters,
+                               void* stream, uint64_t* out3);
+
 const char* chm_jacobi2d_last_error(void);
 
 /* Measured HBM ceiling: device time of one streaming copy of `bytes` bytes
