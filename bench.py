@@ -75,15 +75,18 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles():
+def traffic_from_profiles(depth, band):
     """HBM bytes per launch of the persistent scheduler, measured with
     rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and calibrated against a copy
-    kernel of known size (profiles/pmc_traffic.json), or None."""
+    kernel of known size (profiles/pmc_traffic.json), or None when that
+    profile was taken on another workload, halo depth or band height."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     try:
         with open(path) as f:
             d = json.load(f)
-        if d.get("workload") == f"{W}x{H}/{CHARES[0]}x{CHARES[1]}/{ITERS}":
+        if (d.get("workload") == f"{W}x{H}/{CHARES[0]}x{CHARES[1]}/{ITERS}"
+                and d.get("halo_depth") == depth
+                and d.get("band_rows") == band):
             return d.get("bytes_per_launch")
     except (OSError, ValueError):
         pass
@@ -402,9 +405,8 @@ def main():
     else:
         depth = 2
     min_bytes = BYTES_PER_CELL / depth * cells
-    # the counters were taken on the default configuration only
-    default_cfg = not (args.depth or args.band or nc or u)
-    traffic = traffic_from_profiles() if default_cfg else None
+    # counters exist for one configuration (the profile names it)
+    traffic = traffic_from_profiles(depth, args.band) if not (nc or u) else None
     kavg = statistics.mean(kern_ms)
     achieved = min_bytes / (kavg / 1e3) / 1e9
 

@@ -14,7 +14,8 @@ import weakref
 import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIBDIR = os.path.join(PKG, "lib")
+# CHM_LIBDIR: load the native libraries from another build directory
+LIBDIR = os.environ.get("CHM_LIBDIR") or os.path.join(PKG, "lib")
 
 FORM_PLAIN, FORM_GRAPH, FORM_PERSISTENT = 0, 1, 2
 INIT_SEEDED, INIT_REFERENCE = 0, 1

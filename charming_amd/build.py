@@ -141,6 +141,25 @@ CHARE_PROGRAMS = ["jacobi2d/persistent.hip", "examples/hello.hip",
                   "examples/hello.hip@gpu", "examples/pingpong.hip@gpu"]
 
 
+_ASM = {}
+
+
+def _device_asm(src):
+    """The device assembly of a chare program (compiled once per process)."""
+    import tempfile
+    if src not in _ASM:
+        rel, extra = _split(src)
+        with tempfile.TemporaryDirectory() as d:
+            out = os.path.join(d, "k.s")
+            cmd = [HIPCC] + DEVICE_FLAGS + extra + [
+                "-S", "--cuda-device-only", os.path.join(CSRC, rel), "-o", out]
+            p = subprocess.run(cmd, capture_output=True, text=True)
+            if p.returncode != 0:
+                raise RuntimeError(f"hipcc -S failed on {src}:\n{p.stderr}")
+            _ASM[src] = open(out).read()
+    return _ASM[src]
+
+
 def stack_need(src):
     """Bytes of per-lane stack the deepest call chain of a chare program's
     scheduler kernel needs: the kernel's own chain down to the dispatch of
@@ -148,16 +167,7 @@ def stack_need(src):
     function pointer, so the compiler gives the kernel a dynamic stack and
     cannot size it).  Computed from the frame sizes in the device assembly."""
     import re
-    import tempfile
-    rel, extra = _split(src)
-    with tempfile.TemporaryDirectory() as d:
-        out = os.path.join(d, "k.s")
-        cmd = [HIPCC] + DEVICE_FLAGS + extra + ["-S", "--cuda-device-only",
-                                                os.path.join(CSRC, rel), "-o", out]
-        p = subprocess.run(cmd, capture_output=True, text=True)
-        if p.returncode != 0:
-            raise RuntimeError(f"hipcc -S failed on {src}:\n{p.stderr}")
-        asm = open(out).read()
+    asm = _device_asm(src)
     defs = dict(re.findall(r"\.set (\S+)\.private_seg_size, (.+)", asm))
     memo = {}
 
@@ -171,6 +181,31 @@ def stack_need(src):
     kernel = max(size(n) for n in defs if "scheduler_kernel" in n)
     entries = [size(n) for n in defs if "entry_method" in n]
     return kernel + max(entries, default=0)
+
+
+def kernel_registers(src):
+    """(threads per PE, vector registers per lane) of a chare program's
+    scheduler kernel, from the code object metadata in its device assembly.
+    The count is the unified total (architectural + accumulation
+    registers): the compiler adds the accumulation registers that functions
+    holding indirect calls keep across them to the largest callee's count,
+    so a kernel can need more than its workgroup size allows."""
+    import re
+    asm = _device_asm(src)
+    m = re.search(r"\.name:\s+_ZN5charm2rt16scheduler_kernelILi(\d+)EEEvi\n",
+                  asm)
+    if not m:
+        raise RuntimeError(f"no scheduler kernel in {src}")
+    v = re.search(r"\.vgpr_count:\s+(\d+)", asm[m.end():m.end() + 4000])
+    return int(m.group(1)), int(v.group(1))
+
+
+def register_budget(threads):
+    """Vector registers per lane a workgroup of `threads` may use and still
+    be resident on one CU of gfx950 (512 per SIMD lane, four SIMDs, wave64,
+    allocated in granules of 8)."""
+    waves_per_simd = -(-threads // 256)
+    return 512 // waves_per_simd // 8 * 8
 
 
 def build(verbose=False, jobs=None):

@@ -29,7 +29,9 @@
 //     directly: compiled for the 512-thread PE (256 VGPRs per lane) rather
 //     than an entry method's 128, so sixteen fused sweeps fit in registers
 //     and no callee-saved registers are spilled around an indirect call.
+#ifndef CHARMING_PE_THREADS
 #define CHARMING_PE_THREADS 512
+#endif
 __device__ void jacobi_work(void* token);
 #define CHARMING_WORK_HOOK jacobi_work
 #include <charming.h>
@@ -316,6 +318,36 @@ __device__ inline T* wuni(T* p) {
   return (T*)wuni((int64_t)p);
 }
 
+// Input rows each wave of the deep pass keeps in its register ring (3: one
+// row in flight ahead of the step that needs it; 6: four).
+#ifndef CHM_JACOBI_IN_RING
+#define CHM_JACOBI_IN_RING 3
+#endif
+constexpr int kInRing = CHM_JACOBI_IN_RING;
+// 1: the deep pass runs on the skewed wavefront (sweep_panelD_skew: the D
+// stages of a step are independent); 0: the chained pass (sweep_panelD).
+#ifndef CHM_JACOBI_SKEW
+#define CHM_JACOBI_SKEW 0
+#endif
+constexpr bool kSkew = CHM_JACOBI_SKEW != 0;
+#ifndef CHM_JACOBI_SKEW_GROUP
+#define CHM_JACOBI_SKEW_GROUP 2
+#endif
+
+// One band of one window of the deep pass, in the selected schedule.
+template <int D, bool RES>
+__device__ __forceinline__ void deep_window(const double* in, double* out, int64_t P,
+                                   int bw, int bh, int c0, int r0, int r1,
+                                   u128acc& acc, const EdgeOut2& none,
+                                   RingSides rs) {
+  if (kSkew)
+    sweep_panelD_skew<D, RES, false, CHM_JACOBI_SKEW_GROUP>(
+        in, out, P, bw, bh, c0, r0, r1, acc, none, rs);
+  else
+    sweep_panelD<D, RES, false, 0, kInRing>(in, out, P, bw, bh, c0, r0, r1,
+                                            acc, none, rs);
+}
+
 // D >= 3 fused sweeps of the whole block (halos D deep), overlapping
 // windows x bands.  The outgoing edges are packed afterwards (pack_edges):
 // kept out of the sweep, they would cost its inner loop registers it needs
@@ -323,7 +355,7 @@ __device__ inline T* wuni(T* p) {
 template <int D, bool RES>
 __device__ __noinline__ void chare_sweepD(const double* in, double* out,
                                           int64_t P, int bw, int bh,
-                                          RingSides rs, int n_copy) {
+                                          RingSides rs) {
   in = wuni(in);
   out = wuni(out);
   P = wuni(P);
@@ -333,7 +365,6 @@ __device__ __noinline__ void chare_sweepD(const double* in, double* out,
   rs.bottom = wuni((int)rs.bottom);
   rs.left = wuni((int)rs.left);
   rs.right = wuni((int)rs.right);
-  n_copy = wuni(n_copy);
   // (the wave index stays a vector value: moved to a scalar register, the
   // row bookkeeping derived from it raises scalar-register pressure enough
   // to spill in the deeper sweeps, measured slower)
@@ -342,12 +373,12 @@ __device__ __noinline__ void chare_sweepD(const double* in, double* out,
   const int S = (bw + OW - 1) / OW;
   u128acc acc = {0, 0};
   const EdgeOut2 none = {nullptr, nullptr, nullptr, nullptr};
-  // bands of 256 rows for tall chares at up to eight fused sweeps
-  // (measured best on 1024-row chares); shorter chares, and deeper passes
-  // (whose pipeline fill of 2D-2 rows a short band repeats), are cut into
-  // contiguous shares
+  // bands of 256 rows for tall chares (measured best on 1024-row chares:
+  // at sixteen fused sweeps 132.6 ms per 16384^2 solve against 138.9 with
+  // 384-row bands, 143.1 with 192 and 185.4 with contiguous shares);
+  // shorter chares are cut into contiguous shares
   int band = wuni(g_jdesc.tune_band);
-  if (band == 0) band = (bh >= 1024 && D <= 8) ? 256 : -1;
+  if (band == 0) band = bh >= 1024 ? 256 : -1;
   if (band > 0) {
     // bands of `band` rows, windows of a band side by side: waves take the
     // tiles round-robin, so neighbouring windows (which read overlapping
@@ -356,8 +387,8 @@ __device__ __noinline__ void chare_sweepD(const double* in, double* out,
     for (int t = wave; t < S * nb; t += WV) {
       const int panel = t % S, b = t / S;
       const int r0 = 1 + b * band, r1 = min(r0 + band, bh + 1);
-      sweep_panelD<D, RES, false>(in, out, P, bw, bh, 1 + panel * OW, r0, r1,
-                                  acc, none, rs, -1, n_copy);
+      deep_window<D, RES>(in, out, P, bw, bh, 1 + panel * OW, r0, r1, acc,
+                          none, rs);
     }
   } else {
     // the S windows x bh rows, window after window, cut into one contiguous
@@ -369,8 +400,8 @@ __device__ __noinline__ void chare_sweepD(const double* in, double* out,
     while (pos < end) {
       const int panel = (int)(pos / bh), r = (int)(pos % bh);
       const int len = (int)min((int64_t)(bh - r), end - pos);
-      sweep_panelD<D, RES, false>(in, out, P, bw, bh, 1 + panel * OW, 1 + r,
-                                  1 + r + len, acc, none, rs, -1, n_copy);
+      deep_window<D, RES>(in, out, P, bw, bh, 1 + panel * OW, 1 + r,
+                          1 + r + len, acc, none, rs);
       pos += len;
     }
   }
@@ -380,40 +411,26 @@ __device__ __noinline__ void chare_sweepD(const double* in, double* out,
   }
 }
 
-// `steps` (3 .. kHaloRows) fused sweeps in one pass: 3 .. 8 by their own
-// pass (112-column windows), 9 .. 16 by the sixteen-deep pass (96-column
-// windows) whose first 16 - steps stages only copy.
+// `steps` (3 .. kHaloRows) fused sweeps in one pass: 3 .. 8 on 112-column
+// windows, 9 .. 16 on 96-column windows.  Every depth has its own
+// instantiation, so no stage carries a run-time copy test (a uniform branch
+// per stage would split the step loop into a basic block per stage and
+// leave the stages' latencies exposed).
 __device__ void sweepD_dispatch(int steps, const double* in, double* out,
                                 int64_t P, int bw, int bh, bool res,
                                 RingSides rs) {
 #define CHM_CASED(D_)                                            \
   case D_:                                                       \
     if (res)                                                     \
-      chare_sweepD<D_, true>(in, out, P, bw, bh, rs, 0);         \
+      chare_sweepD<D_, true>(in, out, P, bw, bh, rs);            \
     else                                                         \
-      chare_sweepD<D_, false>(in, out, P, bw, bh, rs, 0);        \
+      chare_sweepD<D_, false>(in, out, P, bw, bh, rs);           \
     break;
   switch (steps) {
     CHM_CASED(3) CHM_CASED(4) CHM_CASED(5) CHM_CASED(6) CHM_CASED(7)
-    CHM_CASED(8)
-    default:
-      if (steps > 8 && steps <= 12) {
-        if (res)
-          chare_sweepD<12, true>(in, out, P, bw, bh, rs, 12 - steps);
-        else
-          chare_sweepD<12, false>(in, out, P, bw, bh, rs, 12 - steps);
-      } else if (steps > 12 && steps <= 14) {
-        if (res)
-          chare_sweepD<14, true>(in, out, P, bw, bh, rs, 14 - steps);
-        else
-          chare_sweepD<14, false>(in, out, P, bw, bh, rs, 14 - steps);
-      } else if (steps > 14 && steps <= 16) {
-        if (res)
-          chare_sweepD<16, true>(in, out, P, bw, bh, rs, 16 - steps);
-        else
-          chare_sweepD<16, false>(in, out, P, bw, bh, rs, 16 - steps);
-      }
-      break;
+    CHM_CASED(8) CHM_CASED(9) CHM_CASED(10) CHM_CASED(11) CHM_CASED(12)
+    CHM_CASED(13) CHM_CASED(14) CHM_CASED(15) CHM_CASED(16)
+    default: break;
   }
 #undef CHM_CASED
 }

@@ -705,16 +705,17 @@ struct PanelW {
 // starts at column c0 (odd): reads the input rows r0-D .. r1+D-1 (columns
 // c0-H .. c0-H+127, those up to w+D), writes the output rows r0 .. r1-1,
 // columns c0 .. min(c0+window_out_cols(D)-1, cend).  The input rows sit in
-// a ring of three register slots (one row in flight), and each intermediate
-// grid keeps its three live rows in a ring of its own; ring slots are
-// static indices because the step loop is unrolled by three.  RES adds the
-// exact residual of the last sweep.  n_copy (0 .. D-1) leading stages only
-// copy their input, so one instantiation also does D - n_copy sweeps (the
-// halos read stay D deep).  RALL = 1 (with RES): stage s = 1 .. D-1 also
-// adds its sweep's exact residual over the stored cells to accs[s-1] (acc
-// is the last stage's); RALL = 2: every stage s = 1 .. D adds the FP64 sum
-// of its squared changes over the stored cells to accd[s-1].
-template <int D, bool RES, bool PACK, int RALL = 0>
+// a ring of RIN register slots (RIN - 2 rows in flight ahead of the step
+// that first needs them), and each intermediate grid keeps its three live
+// rows in a ring of its own; ring slots are static indices because the
+// step loop is unrolled by lcm(3, RIN).  RES adds the exact residual of the
+// last sweep.  n_copy (0 .. D-1) leading stages only copy their input, so
+// one instantiation also does D - n_copy sweeps (the halos read stay D
+// deep).  RALL = 1 (with RES): stage s = 1 .. D-1 also adds its sweep's
+// exact residual over the stored cells to accs[s-1] (acc is the last
+// stage's); RALL = 2: every stage s = 1 .. D adds the FP64 sum of its
+// squared changes over the stored cells to accd[s-1].
+template <int D, bool RES, bool PACK, int RALL = 0, int RIN = 3>
 __device__ inline void sweep_panelD(const double* __restrict__ in,
                                     double* __restrict__ out, int64_t pitch,
                                     int w, int h, int c0, int r0, int r1,
@@ -725,50 +726,59 @@ __device__ inline void sweep_panelD(const double* __restrict__ in,
   static_assert(RALL != 1 || RES, "per-sweep residuals include the last one");
   static_assert(RALL != 2 || !RES, "the FP64 sums replace the exact ones");
   static_assert(D >= 3 && D <= 16, "deep fused sweep: 3 <= D <= 16");
+  static_assert(RIN == 3 || RIN == 6, "input ring of 3 or 6 rows");
+  constexpr int U = RIN == 3 ? 3 : 6;  // lcm(3, RIN)
   if (r0 >= r1) return;
   PanelW p;
   p.setup(in, out, pitch, w, h, c0, cend < 0 ? w : cend, rs, D);
-  RowW R[3];          // input rows
+  RowW R[RIN];        // input rows: row r0-D+i in slot i % RIN
   RowW S[D - 1][3];   // S[s-1]: rows of the grid after s sweeps
+  const int last_in = r1 + D - 1;  // the last input row of the band
 #pragma unroll
-  for (int k = 0; k < 3; k++) {
+  for (int k = 0; k < RIN; k++) {
     R[k].v = make_double2(0.0, 0.0);
-    if (r0 - D + k <= r1 + D - 1) p.load(R[k], r0 - D + k);
+    if (r0 - D + k <= last_in) p.load(R[k], r0 - D + k);
   }
   // step t: the first sweep gives row q1 = r0-D+1+t from input rows q1-1
-  // (slot t % 3), q1, q1+1; sweep s >= 2 gives row q1-(s-1) once t >= 2(s-1)
-  // from rows made at steps t-2, t-1, t; the last one is stored
+  // (slot t % RIN), q1, q1+1; sweep s >= 2 gives row q1-(s-1) once
+  // t >= 2(s-1) from rows made at steps t-2, t-1, t; the last one is
+  // stored.  After step t, input row q1-1 is dead and its slot takes row
+  // q1-1+RIN.
   const int n_steps = r1 - r0 + 2 * D - 2;
   // Steps [tA, tB) form the steady state: every sweep is under way, no row
   // of the grid's ring is met (the ring rows 0 and h+1 only occur among the
   // first 2D-2 and the last D-1 steps) and no load runs past the band, so
   // those steps run without guards or ring checks unless the wave holds a
-  // ring column.  Both bounds are multiples of three, as the ring slots
+  // ring column.  Both bounds are multiples of U, as the ring slots
   // require.
-  const int tA = (2 * D - 2 + 2) / 3 * 3;
-  int tB = tA + (n_steps - (D - 1) - tA) / 3 * 3;
+  const int tA = (2 * D - 2 + U - 1) / U * U;
+  const int t_hi = min(n_steps - (D - 1), n_steps + 2 - RIN);
+  int tB = tA + (t_hi - tA) / U * U;
   if (p.ringcols || tB < tA) tB = tA;
   auto guarded = [&](int t0) {
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
+    for (int k = 0; k < U; k++) {
       const int t = t0 + k;
+      const int ks = k % 3;
       if (t < n_steps) {
         const int q1 = r0 - D + 1 + t;
-        p.template stage<true, RALL>(R[k], R[(k + 1) % 3], R[(k + 2) % 3], q1,
-                                     S[0][k], n_copy >= 1, accs, accd, r0, r1);
-        if (q1 + 2 <= r1 + D - 1) p.load(R[k], q1 + 2);
+        p.template stage<true, RALL>(R[k % RIN], R[(k + 1) % RIN],
+                                     R[(k + 2) % RIN], q1, S[0][ks],
+                                     n_copy >= 1, accs, accd, r0, r1);
+        if (q1 - 1 + RIN <= last_in) p.load(R[k % RIN], q1 - 1 + RIN);
 #pragma unroll
         for (int s = 2; s < D; s++)
           if (t >= 2 * (s - 1))
-            p.template stage<true, RALL>(S[s - 2][(k + 1) % 3],
-                                         S[s - 2][(k + 2) % 3], S[s - 2][k],
-                                         q1 - (s - 1), S[s - 1][k], n_copy >= s,
+            p.template stage<true, RALL>(S[s - 2][(ks + 1) % 3],
+                                         S[s - 2][(ks + 2) % 3], S[s - 2][ks],
+                                         q1 - (s - 1), S[s - 1][ks],
+                                         n_copy >= s,
                                          RALL == 1 ? accs + (s - 1) : nullptr,
                                          RALL == 2 ? accd + (s - 1) : nullptr,
                                          r0, r1);
         if (t >= 2 * (D - 1))
           p.template outrow<D, RES, PACK, RALL == 2>(
-              S[D - 2][(k + 1) % 3], S[D - 2][(k + 2) % 3], S[D - 2][k],
+              S[D - 2][(ks + 1) % 3], S[D - 2][(ks + 2) % 3], S[D - 2][ks],
               q1 - (D - 1), acc, eo, RALL == 2 ? accd + (D - 1) : nullptr);
       }
     }
@@ -777,26 +787,132 @@ __device__ inline void sweep_panelD(const double* __restrict__ in,
   // the guarded steps, so their code exists once)
   int t0 = 0;
   for (;;) {
+    for (; t0 < n_steps && !(t0 >= tA && t0 < tB); t0 += U) guarded(t0);
+    if (t0 >= n_steps) break;
+    for (; t0 < tB; t0 += U) {
+#pragma unroll
+      for (int k = 0; k < U; k++) {
+        const int ks = k % 3;
+        const int q1 = r0 - D + 1 + t0 + k;
+        p.template stage<false, RALL>(R[k % RIN], R[(k + 1) % RIN],
+                                      R[(k + 2) % RIN], q1, S[0][ks],
+                                      n_copy >= 1, accs, accd, r0, r1);
+        p.load(R[k % RIN], q1 - 1 + RIN);
+#pragma unroll
+        for (int s = 2; s < D; s++)
+          p.template stage<false, RALL>(
+              S[s - 2][(ks + 1) % 3], S[s - 2][(ks + 2) % 3], S[s - 2][ks],
+              q1 - (s - 1), S[s - 1][ks], n_copy >= s,
+              RALL == 1 ? accs + (s - 1) : nullptr,
+              RALL == 2 ? accd + (s - 1) : nullptr, r0, r1);
+        p.template outrow<D, RES, PACK, RALL == 2>(
+            S[D - 2][(ks + 1) % 3], S[D - 2][(ks + 2) % 3], S[D - 2][ks],
+            q1 - (D - 1), acc, eo, RALL == 2 ? accd + (D - 1) : nullptr);
+      }
+    }
+  }
+}
+
+// The same D fused sweeps on a skewed wavefront: at step t, sweep s makes
+// row q1 - 2(s-1) (q1 = r0-D+1+t, the row the first sweep makes) from the
+// rows sweep s-1 made at steps t-3, t-2, t-1, never from one made at step
+// t.  So the D stages of a step are independent of each other (the
+// unskewed pass chains them: stage s needs the row stage s-1 has just
+// made), and a wave has D stages of work to interleave instead of one
+// chain.  The stages of a step run from the last to the first: stage s
+// reads stage s-1's slot t % 3 (its row of step t-3) before stage s-1
+// overwrites that slot with its row of step t, so each intermediate grid
+// still keeps three rows.  The pipeline is 3(D-1) steps deep instead of
+// 2(D-1): r1 - r0 + 3D - 3 steps per band.  Same arithmetic in the same
+// order as D single sweeps, so the result is bit-identical.
+template <int D, bool RES, bool PACK, int G = 2>
+__device__ inline void sweep_panelD_skew(const double* __restrict__ in,
+                                         double* __restrict__ out,
+                                         int64_t pitch, int w, int h, int c0,
+                                         int r0, int r1, u128acc& acc,
+                                         const EdgeOut2& eo, RingSides rs,
+                                         int cend = -1) {
+  static_assert(D >= 3 && D <= 16, "deep fused sweep: 3 <= D <= 16");
+  if (r0 >= r1) return;
+  PanelW p;
+  p.setup(in, out, pitch, w, h, c0, cend < 0 ? w : cend, rs, D);
+  RowW R[3];          // input rows: row r0-D+i in slot i % 3
+  RowW S[D - 1][3];   // S[s-1]: rows of the grid after s sweeps, made at
+                      // step t in slot t % 3
+  const int last_in = r1 + D - 1;  // the last input row of the band
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    R[k].v = make_double2(0.0, 0.0);
+    if (r0 - D + k <= last_in) p.load(R[k], r0 - D + k);
+  }
+  const int n_rows = r1 - r0;
+  const int n_steps = n_rows + 3 * D - 3;
+  // sweep s makes its rows at steps 3(s-1) .. n_rows + 2D - 4 + s (rows
+  // r0-D+s .. r1+D-1-s), the last sweep (stored) at 3(D-1) .. n_steps-1
+  auto active = [&](int s, int t) {
+    return t >= 3 * (s - 1) && t <= n_rows + 2 * D - 4 + s;
+  };
+  // Steps [tA, tB) form the steady state: every sweep is under way, the
+  // loads stay within the band and no row of the grid's ring is made (the
+  // top ring row only among the first 3D-3 steps; the bottom one, made
+  // first by sweep 1 at step h - r0 + D, bounds tB), so those steps run
+  // without guards or ring checks unless the wave holds a ring column.
+  // Both bounds are multiples of three, as the ring slots require.
+  const int tA = (3 * D - 3 + 2) / 3 * 3;
+  int t_hi = n_rows + 2 * D - 3;  // the first sweep's last step + 1
+  if (rs.bottom) t_hi = min(t_hi, h - r0 + D);
+  int tB = tA + (t_hi - tA) / 3 * 3;
+  if (p.ringcols || tB < tA) tB = tA;
+  auto guarded = [&](int t0) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const int t = t0 + k;
+      if (t < n_steps) {
+        const int q1 = r0 - D + 1 + t;
+        if (t >= 3 * (D - 1))
+          p.template outrow<D, RES, PACK>(S[D - 2][k], S[D - 2][(k + 1) % 3],
+                                          S[D - 2][(k + 2) % 3],
+                                          q1 - 2 * (D - 1), acc, eo);
+#pragma unroll
+        for (int s = D - 1; s >= 2; s--)
+          if (active(s, t))
+            p.template stage<true>(S[s - 2][k], S[s - 2][(k + 1) % 3],
+                                   S[s - 2][(k + 2) % 3], q1 - 2 * (s - 1),
+                                   S[s - 1][k]);
+        if (active(1, t)) {
+          p.template stage<true>(R[k], R[(k + 1) % 3], R[(k + 2) % 3], q1,
+                                 S[0][k]);
+          if (q1 + 2 <= last_in) p.load(R[k], q1 + 2);
+        }
+      }
+    }
+  };
+  int t0 = 0;
+  for (;;) {
     for (; t0 < n_steps && !(t0 >= tA && t0 < tB); t0 += 3) guarded(t0);
     if (t0 >= n_steps) break;
     for (; t0 < tB; t0 += 3) {
 #pragma unroll
       for (int k = 0; k < 3; k++) {
         const int q1 = r0 - D + 1 + t0 + k;
-        p.template stage<false, RALL>(R[k], R[(k + 1) % 3], R[(k + 2) % 3],
-                                      q1, S[0][k], n_copy >= 1, accs, accd, r0,
-                                      r1);
-        p.load(R[k], q1 + 2);
+        p.template outrow<D, RES, PACK>(S[D - 2][k], S[D - 2][(k + 1) % 3],
+                                        S[D - 2][(k + 2) % 3],
+                                        q1 - 2 * (D - 1), acc, eo);
 #pragma unroll
-        for (int s = 2; s < D; s++)
-          p.template stage<false, RALL>(
-              S[s - 2][(k + 1) % 3], S[s - 2][(k + 2) % 3], S[s - 2][k],
-              q1 - (s - 1), S[s - 1][k], n_copy >= s,
-              RALL == 1 ? accs + (s - 1) : nullptr,
-              RALL == 2 ? accd + (s - 1) : nullptr, r0, r1);
-        p.template outrow<D, RES, PACK, RALL == 2>(
-            S[D - 2][(k + 1) % 3], S[D - 2][(k + 2) % 3], S[D - 2][k],
-            q1 - (D - 1), acc, eo, RALL == 2 ? accd + (D - 1) : nullptr);
+        for (int s = D - 1; s >= 2; s--) {
+          // groups of G stages: the compiler interleaves a group's
+          // independent stages, and a scheduling barrier keeps it from
+          // hoisting more (the registers of every stage in flight at once
+          // would not fit)
+          if ((D - s) % G == 0) __builtin_amdgcn_sched_barrier(0);
+          p.template stage<false>(S[s - 2][k], S[s - 2][(k + 1) % 3],
+                                  S[s - 2][(k + 2) % 3], q1 - 2 * (s - 1),
+                                  S[s - 1][k]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        p.template stage<false>(R[k], R[(k + 1) % 3], R[(k + 2) % 3], q1,
+                                S[0][k]);
+        p.load(R[k], q1 + 2);
       }
     }
   }

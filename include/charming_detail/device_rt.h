@@ -627,19 +627,30 @@ __device__ __noinline__ void run_work() {
 #endif
 }
 
-__device__ __noinline__ void pe_loop() {
+// One turn of the scheduler: wait for work and run it; false once the PE
+// is told to stop.
+__device__ __noinline__ bool pe_step() {
   Lds& L = lds();
-  for (;;) {
-    if (threadIdx.x == 0) L.cmd = poll_next();
-    __syncthreads();
-    const int cmd = L.cmd;
-    if (cmd == CMD_EXIT) break;
-    if (cmd == CMD_INBOX)
-      process_inbox();
-    else if (cmd == CMD_REVIVED)
-      process_revived();
-    else
-      process_async();
+  if (threadIdx.x == 0) L.cmd = poll_next();
+  __syncthreads();
+  const int cmd = L.cmd;
+  if (cmd == CMD_EXIT) return false;
+  if (cmd == CMD_INBOX)
+    process_inbox();
+  else if (cmd == CMD_REVIVED)
+    process_revived();
+  else
+    process_async();
+  return true;
+}
+
+// The scheduler loop.  The work hook is called from here, between turns,
+// where nothing is held in vector registers across the call: the hook may
+// use every register the workgroup size allows, and a value this loop kept
+// across it would be parked in accumulation registers, which count towards
+// the kernel's register total on top of the hook's.
+__device__ __noinline__ void pe_loop() {
+  while (pe_step()) {
     run_work();
     __syncthreads();
   }

@@ -1,16 +1,17 @@
 """CPU model of the deep fused sweep's index logic (overlapping windows of
 64 lanes x 2 columns, D sweeps per pass, ring cells kept, guarded and steady
-steps), checked against D single sweeps of the oracle on chare blocks cut
-from a global grid.  The model restates the device code's loop and window
-arithmetic step by step, so a wrong window offset, ring column, store range,
-step split or band split shows up here before any GPU run."""
+steps, an input ring of 3 or 6 rows, leading copy stages), checked against
+D single sweeps of the oracle on chare blocks cut from a global grid.  The
+model restates the device code's loop and window arithmetic step by step,
+so a wrong window offset, ring column, store range, step split, input-ring
+slot or band split shows up here before any GPU run."""
 import numpy as np
 import pytest
 
 from oracle import oracle as O
 
 K_COL_OFF = 15   # element offset of halo column 0 within a row
-K_HALO = 8       # halo rows/columns every chare array holds
+K_HALO = 16      # halo rows/columns every chare array holds
 
 
 def pitch_for_depth(w, d):
@@ -30,7 +31,9 @@ def shl1z(v):
 
 
 def window_margin(D):
-    return 8   # D <= 8: 112 output columns, seven whole 128-byte lines
+    # D <= 8: 112 output columns (seven whole 128-byte lines); 9 <= D <= 16:
+    # 96 (six)
+    return 8 if D <= 8 else 16
 
 
 def window_out_cols(D):
@@ -38,7 +41,7 @@ def window_out_cols(D):
 
 
 class Chare:
-    """A chare array: rows 1-K_HALO .. bh+K_HALO, pitch_for_depth(bw, 8)."""
+    """A chare array: rows 1-K_HALO .. bh+K_HALO, pitch_for_depth(bw, 16)."""
 
     def __init__(self, bw, bh):
         self.bw, self.bh = bw, bh
@@ -49,8 +52,12 @@ class Chare:
         return row + K_HALO - 1, K_COL_OFF + col
 
 
-def sweep_window(D, inp, out, w, h, c0, r0, r1, rs, acc, eo, pack):
-    """sweep_panelD<D, RES, PACK> on one band of one window."""
+def sweep_window(D, inp, out, w, h, c0, r0, r1, rs, acc, eo, pack, rin=3,
+                 n_copy=0):
+    """sweep_panelD<D, RES, PACK, 0, rin> on one band of one window, the
+    first n_copy stages copying."""
+    assert rin in (3, 6)
+    U = 3 if rin == 3 else 6
     H = window_margin(D)
     lane = np.arange(64)
     c = c0 - H + 2 * lane
@@ -63,6 +70,116 @@ def sweep_window(D, inp, out, w, h, c0, r0, r1, rs, acc, eo, pack):
     cend = w
 
     def load(r, row):
+        # never past the band's input rows, never outside the array
+        assert r0 - D <= row <= r1 + D - 1
+        assert 1 - K_HALO <= row <= inp.bh + K_HALO
+        i, j = inp.idx(row, c[vload])
+        r[vload, 0] = inp.a[i, j]
+        r[vload, 1] = inp.a[i, j + 1]
+
+    def stage(up, cu, dn, q, check, copy=False):
+        if copy or (check and ((q == 0 and top) or (q == h + 1 and bottom))):
+            return cu.copy()
+        a, b = cu[:, 0], cu[:, 1]
+        na = ((((a + shr1z(b)) + b) + up[:, 0]) + dn[:, 0]) * 0.2
+        nb = ((((b + a) + shl1z(a)) + up[:, 1]) + dn[:, 1]) * 0.2
+        if check and ringcols:
+            na = np.where(ring_a, a, na)
+            nb = np.where(ring_b, b, nb)
+        return np.stack([na, nb], axis=1)
+
+    def outrow(up, cu, dn, row):
+        a, b = cu[:, 0], cu[:, 1]
+        na = ((((a + shr1z(b)) + b) + up[:, 0]) + dn[:, 0]) * 0.2
+        nb = ((((b + a) + shl1z(a)) + up[:, 1]) + dn[:, 1]) * 0.2
+        for l in np.nonzero(vstore)[0]:
+            cl = c[l]
+            cells = [(cl, na[l], a[l])] if cl <= cend else []
+            if cl + 1 <= cend:
+                cells.append((cl + 1, nb[l], b[l]))
+            for col, v, old in cells:
+                i, j = out.idx(row, col)
+                out.a[i, j] = v
+                acc.append((v, old))
+                if pack:
+                    eo.record(row, col, v)
+
+    R = [np.zeros((64, 2)) for _ in range(rin)]
+    last_in = r1 + D - 1
+    for k in range(rin):
+        if r0 - D + k <= last_in:
+            load(R[k], r0 - D + k)
+    S = [[np.zeros((64, 2)) for _ in range(3)] for _ in range(D - 1)]
+    n_steps = r1 - r0 + 2 * D - 2
+    tA = (2 * D - 2 + U - 1) // U * U
+    t_hi = min(n_steps - (D - 1), n_steps + 2 - rin)
+    x = t_hi - tA
+    tB = tA + int(x / U) * U   # C's division truncates toward zero
+    if ringcols or tB < tA:
+        tB = tA
+    t0 = 0
+    while True:
+        while t0 < n_steps and not (tA <= t0 < tB):
+            for k in range(U):
+                t, ks = t0 + k, k % 3
+                if t < n_steps:
+                    q1 = r0 - D + 1 + t
+                    S[0][ks] = stage(R[k % rin], R[(k + 1) % rin],
+                                     R[(k + 2) % rin], q1, True, n_copy >= 1)
+                    if q1 - 1 + rin <= last_in:
+                        load(R[k % rin], q1 - 1 + rin)
+                    for s in range(2, D):
+                        if t >= 2 * (s - 1):
+                            S[s - 1][ks] = stage(S[s - 2][(ks + 1) % 3],
+                                                 S[s - 2][(ks + 2) % 3],
+                                                 S[s - 2][ks], q1 - (s - 1),
+                                                 True, n_copy >= s)
+                    if t >= 2 * (D - 1):
+                        outrow(S[D - 2][(ks + 1) % 3], S[D - 2][(ks + 2) % 3],
+                               S[D - 2][ks], q1 - (D - 1))
+            t0 += U
+        if t0 >= n_steps:
+            break
+        while t0 < tB:
+            for k in range(U):
+                ks = k % 3
+                q1 = r0 - D + 1 + t0 + k
+                # the steady state must never meet what it skips
+                assert not ((q1 == 0 and top) or (q1 == h + 1 and bottom))
+                S[0][ks] = stage(R[k % rin], R[(k + 1) % rin], R[(k + 2) % rin],
+                                 q1, False, n_copy >= 1)
+                load(R[k % rin], q1 - 1 + rin)
+                for s in range(2, D):
+                    q = q1 - (s - 1)
+                    assert not ((q == 0 and top) or (q == h + 1 and bottom))
+                    S[s - 1][ks] = stage(S[s - 2][(ks + 1) % 3],
+                                         S[s - 2][(ks + 2) % 3], S[s - 2][ks],
+                                         q, False, n_copy >= s)
+                outrow(S[D - 2][(ks + 1) % 3], S[D - 2][(ks + 2) % 3],
+                       S[D - 2][ks], q1 - (D - 1))
+            t0 += U
+
+
+def sweep_window_skew(D, inp, out, w, h, c0, r0, r1, rs, acc, eo, pack):
+    """sweep_panelD_skew<D, RES, PACK> on one band of one window: sweep s
+    makes row q1 - 2(s-1) at step t from rows sweep s-1 made at steps t-3,
+    t-2, t-1; the stages of a step run from the last to the first."""
+    H = window_margin(D)
+    lane = np.arange(64)
+    c = c0 - H + 2 * lane
+    vload = c <= w + D
+    vstore = (lane >= H // 2) & (lane < 64 - H // 2)
+    top, bottom, left, right = rs
+    ring_a = (left & (c == 0)) | (right & (c == w + 1))
+    ring_b = (left & (c + 1 == 0)) | (right & (c + 1 == w + 1))
+    ringcols = bool(np.any(ring_a | ring_b))
+    cend = w
+    last_in = r1 + D - 1
+    made = {}   # (sweep, row) -> step, to check what each stage reads
+
+    def load(r, row):
+        assert r0 - D <= row <= last_in
+        assert 1 - K_HALO <= row <= inp.bh + K_HALO
         i, j = inp.idx(row, c[vload])
         r[vload, 0] = inp.a[i, j]
         r[vload, 1] = inp.a[i, j + 1]
@@ -96,52 +213,71 @@ def sweep_window(D, inp, out, w, h, c0, r0, r1, rs, acc, eo, pack):
 
     R = [np.zeros((64, 2)) for _ in range(3)]
     for k in range(3):
-        if r0 - D + k <= r1 + D - 1:
+        if r0 - D + k <= last_in:
             load(R[k], r0 - D + k)
     S = [[np.zeros((64, 2)) for _ in range(3)] for _ in range(D - 1)]
-    n_steps = r1 - r0 + 2 * D - 2
-    tA = (2 * D - 2 + 2) // 3 * 3
-    x = n_steps - (D - 1) - tA
-    tB = tA + int(x / 3) * 3   # C's division truncates toward zero
+    n_rows = r1 - r0
+    n_steps = n_rows + 3 * D - 3
+
+    def active(s, t):
+        return 3 * (s - 1) <= t <= n_rows + 2 * D - 4 + s
+
+    tA = (3 * D - 3 + 2) // 3 * 3
+    t_hi = n_rows + 2 * D - 3
+    if bottom:
+        t_hi = min(t_hi, h - r0 + D)
+    tB = tA + int((t_hi - tA) / 3) * 3   # C's division truncates toward zero
     if ringcols or tB < tA:
         tB = tA
+
+    def reads(s, q, t):
+        # rows q-1, q, q+1 of sweep s-1 (key s-2), each made at an earlier
+        # step (rows of the input, s = 1, are loaded, not made)
+        for d in (-1, 0, 1):
+            assert made.get((s - 2, q + d), -1) < t, (s, q + d, t)
+            assert (s - 2, q + d) in made, (s, q + d, t)
+
+    def step(t, guarded):
+        k = t % 3
+        q1 = r0 - D + 1 + t
+        if not guarded:
+            for s in range(1, D + 1):
+                q = q1 - 2 * (s - 1)
+                # the steady state must never meet what it skips
+                assert active(s, t)
+                assert not ((q == 0 and top) or (q == h + 1 and bottom))
+        if t >= 3 * (D - 1):
+            reads(D, q1 - 2 * (D - 1), t)
+            outrow(S[D - 2][k], S[D - 2][(k + 1) % 3], S[D - 2][(k + 2) % 3],
+                   q1 - 2 * (D - 1))
+        for s in range(D - 1, 1, -1):
+            if not guarded or active(s, t):
+                q = q1 - 2 * (s - 1)
+                reads(s, q, t)
+                S[s - 1][k] = stage(S[s - 2][k], S[s - 2][(k + 1) % 3],
+                                    S[s - 2][(k + 2) % 3], q, guarded)
+                made[(s - 1, q)] = t
+        if not guarded or active(1, t):
+            S[0][k] = stage(R[k], R[(k + 1) % 3], R[(k + 2) % 3], q1, guarded)
+            made[(0, q1)] = t
+            if guarded:
+                if q1 + 2 <= last_in:
+                    load(R[k], q1 + 2)
+            else:
+                load(R[k], q1 + 2)
+
     t0 = 0
     while True:
         while t0 < n_steps and not (tA <= t0 < tB):
             for k in range(3):
-                t = t0 + k
-                if t < n_steps:
-                    q1 = r0 - D + 1 + t
-                    S[0][k] = stage(R[k], R[(k + 1) % 3], R[(k + 2) % 3], q1, True)
-                    if q1 + 2 <= r1 + D - 1:
-                        load(R[k], q1 + 2)
-                    for s in range(2, D):
-                        if t >= 2 * (s - 1):
-                            S[s - 1][k] = stage(S[s - 2][(k + 1) % 3],
-                                                S[s - 2][(k + 2) % 3],
-                                                S[s - 2][k], q1 - (s - 1), True)
-                    if t >= 2 * (D - 1):
-                        outrow(S[D - 2][(k + 1) % 3], S[D - 2][(k + 2) % 3],
-                               S[D - 2][k], q1 - (D - 1))
+                if t0 + k < n_steps:
+                    step(t0 + k, True)
             t0 += 3
         if t0 >= n_steps:
             break
         while t0 < tB:
             for k in range(3):
-                q1 = r0 - D + 1 + t0 + k
-                # the steady state must never meet what it skips
-                assert not ((q1 == 0 and top) or (q1 == h + 1 and bottom))
-                assert q1 + 2 <= r1 + D - 1
-                S[0][k] = stage(R[k], R[(k + 1) % 3], R[(k + 2) % 3], q1, False)
-                load(R[k], q1 + 2)
-                for s in range(2, D):
-                    q = q1 - (s - 1)
-                    assert not ((q == 0 and top) or (q == h + 1 and bottom))
-                    S[s - 1][k] = stage(S[s - 2][(k + 1) % 3],
-                                        S[s - 2][(k + 2) % 3], S[s - 2][k], q,
-                                        False)
-                outrow(S[D - 2][(k + 1) % 3], S[D - 2][(k + 2) % 3],
-                       S[D - 2][k], q1 - (D - 1))
+                step(t0 + k, False)
             t0 += 3
 
 
@@ -167,7 +303,8 @@ class Edges:
             self.right[col - (w - D + 1), row - 1] = v
 
 
-def chare_sweep(D, inp, out, rs, waves=16, band=0):
+def chare_sweep(D, inp, out, rs, waves=16, band=0, rin=3, n_copy=0,
+                skew=False):
     """chare_sweepD: with band > 0, bands of `band` rows x windows taken
     round-robin; else the windows x rows, window after window, cut into one
     contiguous range per wave (0: bands of 256 rows for chares of 1024+
@@ -183,8 +320,13 @@ def chare_sweep(D, inp, out, rs, waves=16, band=0):
         for t in range(S * nb):
             panel, b = t % S, t // S
             r0 = 1 + b * band
-            sweep_window(D, inp, out, bw, bh, 1 + panel * OW, r0,
-                         min(r0 + band, bh + 1), rs, acc, eo, True)
+            if skew:
+                sweep_window_skew(D, inp, out, bw, bh, 1 + panel * OW, r0,
+                                  min(r0 + band, bh + 1), rs, acc, eo, True)
+            else:
+                sweep_window(D, inp, out, bw, bh, 1 + panel * OW, r0,
+                             min(r0 + band, bh + 1), rs, acc, eo, True, rin,
+                             n_copy)
         return acc, eo
     total = S * bh
     per = (total + waves - 1) // waves
@@ -194,8 +336,12 @@ def chare_sweep(D, inp, out, rs, waves=16, band=0):
         while pos < end:
             panel, r = pos // bh, pos % bh
             n = min(bh - r, end - pos)
-            sweep_window(D, inp, out, bw, bh, 1 + panel * OW, 1 + r, 1 + r + n,
-                         rs, acc, eo, True)
+            if skew:
+                sweep_window_skew(D, inp, out, bw, bh, 1 + panel * OW, 1 + r,
+                                  1 + r + n, rs, acc, eo, True)
+            else:
+                sweep_window(D, inp, out, bw, bh, 1 + panel * OW, 1 + r,
+                             1 + r + n, rs, acc, eo, True, rin, n_copy)
             pos += n
     return acc, eo
 
@@ -215,16 +361,20 @@ def cut(G, W, H, row0, col0, bw, bh):
     return ch
 
 
+@pytest.mark.parametrize("rin", [3, 6])
 @pytest.mark.parametrize("band", [0, 7, 64])
-@pytest.mark.parametrize("D", [3, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("D,n_copy", [(3, 0), (4, 0), (5, 0), (6, 0), (7, 0),
+                                      (8, 0), (12, 0), (12, 3), (16, 0),
+                                      (16, 1)])
 @pytest.mark.parametrize("W,H,cr,cc,which", [
     (8, 8, 1, 1, (0, 0)), (12, 30, 2, 2, (1, 0)),
     (300, 200, 3, 5, (0, 0)), (300, 200, 3, 5, (1, 2)), (300, 200, 3, 5, (2, 4)),
     (260, 70, 1, 1, (0, 0)), (129, 257, 2, 2, (1, 1)), (1000, 40, 1, 3, (0, 2))])
-def test_window_sweep_matches_oracle(D, W, H, cr, cc, which, band):
+def test_window_sweep_matches_oracle(D, n_copy, W, H, cr, cc, which, band,
+                                     rin):
     G = O.init_grid(W, H, seed=W + 7 * H + D)
     ref = G.copy()
-    for _ in range(D):
+    for _ in range(D - n_copy):
         ref = O.step(ref)
     ci, cj = which
     # the last chare row/column takes the rest of an uneven split
@@ -238,7 +388,7 @@ def test_window_sweep_matches_oracle(D, W, H, cr, cc, which, band):
     inp = cut(G, W, H, row0, col0, bw, bh)
     out = Chare(bw, bh)
     rs = (ci == 0, ci == cr - 1, cj == 0, cj == cc - 1)
-    acc, eo = chare_sweep(D, inp, out, rs, band=band)
+    acc, eo = chare_sweep(D, inp, out, rs, band=band, rin=rin, n_copy=n_copy)
     ii, jj = out.idx(np.arange(1, bh + 1)[:, None], np.arange(1, bw + 1)[None, :])
     got = out.a[ii, jj]
     want = ref[row0 + 1: row0 + bh + 1, col0 + 1: col0 + bw + 1]
@@ -265,10 +415,10 @@ def test_model_catches_a_wrong_ring_column():
     global sweep_window
     good = sweep_window
 
-    def bad(D, inp, out, w, h, c0, r0, r1, rs, acc, eo, pack):
+    def bad(D, inp, out, w, h, c0, r0, r1, rs, acc, eo, pack, *rest):
         top, bottom, left, right = rs
         return good(D, inp, out, w, h, c0, r0, r1, (top, bottom, False, right),
-                    acc, eo, pack)
+                    acc, eo, pack, *rest)
     sweep_window = bad
     try:
         chare_sweep(D, inp, out, (True, True, True, True))
@@ -276,3 +426,33 @@ def test_model_catches_a_wrong_ring_column():
         sweep_window = good
     ii, jj = out.idx(np.arange(1, H + 1)[:, None], np.arange(1, W + 1)[None, :])
     assert not np.array_equal(out.a[ii, jj], ref[1:H + 1, 1:W + 1])
+
+
+@pytest.mark.parametrize("band", [0, 7, 64])
+@pytest.mark.parametrize("D", [3, 4, 5, 8, 9, 12, 16])
+@pytest.mark.parametrize("W,H,cr,cc,which", [
+    (8, 8, 1, 1, (0, 0)), (12, 30, 2, 2, (1, 0)),
+    (300, 200, 3, 5, (0, 0)), (300, 200, 3, 5, (1, 2)), (300, 200, 3, 5, (2, 4)),
+    (260, 70, 1, 1, (0, 0)), (129, 257, 2, 2, (1, 1)), (1000, 40, 1, 3, (0, 2))])
+def test_skewed_window_sweep_matches_oracle(D, W, H, cr, cc, which, band):
+    G = O.init_grid(W, H, seed=W + 5 * H + D)
+    ref = G.copy()
+    for _ in range(D):
+        ref = O.step(ref)
+    ci, cj = which
+    bh_base, bw_base = H // cr, W // cc
+    row0, col0 = ci * bh_base, cj * bw_base
+    bh = bh_base if ci < cr - 1 else H - bh_base * (cr - 1)
+    bw = bw_base if cj < cc - 1 else W - bw_base * (cc - 1)
+    if min(bw, bh) < D:
+        pytest.skip("the runtime refuses chares thinner than the halo depth")
+    inp = cut(G, W, H, row0, col0, bw, bh)
+    out = Chare(bw, bh)
+    rs = (ci == 0, ci == cr - 1, cj == 0, cj == cc - 1)
+    acc, eo = chare_sweep(D, inp, out, rs, band=band, skew=True)
+    ii, jj = out.idx(np.arange(1, bh + 1)[:, None], np.arange(1, bw + 1)[None, :])
+    want = ref[row0 + 1: row0 + bh + 1, col0 + 1: col0 + bw + 1]
+    assert np.array_equal(out.a[ii, jj], want)
+    assert len(acc) == bw * bh
+    assert np.array_equal(eo.top, want[:D, :])
+    assert np.array_equal(eo.right, want[:, bw - D:].T)
