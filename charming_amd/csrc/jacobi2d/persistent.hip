@@ -318,10 +318,13 @@ __device__ inline T* wuni(T* p) {
   return (T*)wuni((int64_t)p);
 }
 
-// Input rows each wave of the deep pass keeps in its register ring (3: one
-// row in flight ahead of the step that needs it; 6: four).
+// Input rows each wave of the deep pass keeps in its register ring: 6,
+// four rows in flight ahead of the step that first needs them (with two
+// waves per SIMD, one row in flight left the loads exposed: 16384^2 x 1000
+// at sixteen fused sweeps 123.6 ms with 6 slots against 133.8 ms with 3;
+// 9 and 12 slots, 130-131 ms, raise register pressure more than they hide)
 #ifndef CHM_JACOBI_IN_RING
-#define CHM_JACOBI_IN_RING 3
+#define CHM_JACOBI_IN_RING 6
 #endif
 constexpr int kInRing = CHM_JACOBI_IN_RING;
 // 1: the deep pass runs on the skewed wavefront (sweep_panelD_skew: the D
@@ -330,9 +333,7 @@ constexpr int kInRing = CHM_JACOBI_IN_RING;
 #define CHM_JACOBI_SKEW 0
 #endif
 constexpr bool kSkew = CHM_JACOBI_SKEW != 0;
-#ifndef CHM_JACOBI_SKEW_GROUP
-#define CHM_JACOBI_SKEW_GROUP 2
-#endif
+
 
 // One band of one window of the deep pass, in the selected schedule.
 template <int D, bool RES>
@@ -341,8 +342,8 @@ __device__ __forceinline__ void deep_window(const double* in, double* out, int64
                                    u128acc& acc, const EdgeOut2& none,
                                    RingSides rs) {
   if (kSkew)
-    sweep_panelD_skew<D, RES, false, CHM_JACOBI_SKEW_GROUP>(
-        in, out, P, bw, bh, c0, r0, r1, acc, none, rs);
+    sweep_panelD_skew<D, RES, false, kInRing>(in, out, P, bw, bh, c0, r0, r1,
+                                              acc, none, rs);
   else
     sweep_panelD<D, RES, false, 0, kInRing>(in, out, P, bw, bh, c0, r0, r1,
                                             acc, none, rs);

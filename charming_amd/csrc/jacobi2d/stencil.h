@@ -705,10 +705,10 @@ struct PanelW {
 // starts at column c0 (odd): reads the input rows r0-D .. r1+D-1 (columns
 // c0-H .. c0-H+127, those up to w+D), writes the output rows r0 .. r1-1,
 // columns c0 .. min(c0+window_out_cols(D)-1, cend).  The input rows sit in
-// a ring of RIN register slots (RIN - 2 rows in flight ahead of the step
-// that first needs them), and each intermediate grid keeps its three live
-// rows in a ring of its own; ring slots are static indices because the
-// step loop is unrolled by lcm(3, RIN).  RES adds the exact residual of the
+// a ring of RIN register slots (RIN = 3, 6, 9 or 12: RIN - 2 rows in
+// flight ahead of the step that first needs them), and each intermediate
+// grid keeps its three live rows in a ring of its own; ring slots are
+// static indices because the step loop is unrolled by RIN.  RES adds the exact residual of the
 // last sweep.  n_copy (0 .. D-1) leading stages only copy their input, so
 // one instantiation also does D - n_copy sweeps (the halos read stay D
 // deep).  RALL = 1 (with RES): stage s = 1 .. D-1 also adds its sweep's
@@ -726,8 +726,9 @@ __device__ inline void sweep_panelD(const double* __restrict__ in,
   static_assert(RALL != 1 || RES, "per-sweep residuals include the last one");
   static_assert(RALL != 2 || !RES, "the FP64 sums replace the exact ones");
   static_assert(D >= 3 && D <= 16, "deep fused sweep: 3 <= D <= 16");
-  static_assert(RIN == 3 || RIN == 6, "input ring of 3 or 6 rows");
-  constexpr int U = RIN == 3 ? 3 : 6;  // lcm(3, RIN)
+  static_assert(RIN >= 3 && RIN <= 12 && RIN % 3 == 0,
+                "input ring of 3, 6, 9 or 12 rows");
+  constexpr int U = RIN;  // lcm(3, RIN)
   if (r0 >= r1) return;
   PanelW p;
   p.setup(in, out, pitch, w, h, c0, cend < 0 ? w : cend, rs, D);
@@ -825,7 +826,7 @@ __device__ inline void sweep_panelD(const double* __restrict__ in,
 // still keeps three rows.  The pipeline is 3(D-1) steps deep instead of
 // 2(D-1): r1 - r0 + 3D - 3 steps per band.  Same arithmetic in the same
 // order as D single sweeps, so the result is bit-identical.
-template <int D, bool RES, bool PACK, int G = 2>
+template <int D, bool RES, bool PACK, int RIN = 3>
 __device__ inline void sweep_panelD_skew(const double* __restrict__ in,
                                          double* __restrict__ out,
                                          int64_t pitch, int w, int h, int c0,
@@ -833,15 +834,18 @@ __device__ inline void sweep_panelD_skew(const double* __restrict__ in,
                                          const EdgeOut2& eo, RingSides rs,
                                          int cend = -1) {
   static_assert(D >= 3 && D <= 16, "deep fused sweep: 3 <= D <= 16");
+  static_assert(RIN >= 3 && RIN <= 12 && RIN % 3 == 0,
+                "input ring of 3, 6, 9 or 12 rows");
+  constexpr int U = RIN;  // lcm(3, RIN)
   if (r0 >= r1) return;
   PanelW p;
   p.setup(in, out, pitch, w, h, c0, cend < 0 ? w : cend, rs, D);
-  RowW R[3];          // input rows: row r0-D+i in slot i % 3
+  RowW R[RIN];        // input rows: row r0-D+i in slot i % RIN
   RowW S[D - 1][3];   // S[s-1]: rows of the grid after s sweeps, made at
                       // step t in slot t % 3
   const int last_in = r1 + D - 1;  // the last input row of the band
 #pragma unroll
-  for (int k = 0; k < 3; k++) {
+  for (int k = 0; k < RIN; k++) {
     R[k].v = make_double2(0.0, 0.0);
     if (r0 - D + k <= last_in) p.load(R[k], r0 - D + k);
   }
@@ -857,62 +861,57 @@ __device__ inline void sweep_panelD_skew(const double* __restrict__ in,
   // top ring row only among the first 3D-3 steps; the bottom one, made
   // first by sweep 1 at step h - r0 + D, bounds tB), so those steps run
   // without guards or ring checks unless the wave holds a ring column.
-  // Both bounds are multiples of three, as the ring slots require.
-  const int tA = (3 * D - 3 + 2) / 3 * 3;
-  int t_hi = n_rows + 2 * D - 3;  // the first sweep's last step + 1
+  // Both bounds are multiples of U, as the ring slots require.
+  const int tA = (3 * D - 3 + U - 1) / U * U;
+  int t_hi = n_rows + 2 * D - RIN;  // the last load stays in the band
   if (rs.bottom) t_hi = min(t_hi, h - r0 + D);
-  int tB = tA + (t_hi - tA) / 3 * 3;
+  int tB = tA + (t_hi - tA) / U * U;
   if (p.ringcols || tB < tA) tB = tA;
   auto guarded = [&](int t0) {
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
+    for (int k = 0; k < U; k++) {
       const int t = t0 + k;
+      const int ks = k % 3;
       if (t < n_steps) {
         const int q1 = r0 - D + 1 + t;
         if (t >= 3 * (D - 1))
-          p.template outrow<D, RES, PACK>(S[D - 2][k], S[D - 2][(k + 1) % 3],
-                                          S[D - 2][(k + 2) % 3],
+          p.template outrow<D, RES, PACK>(S[D - 2][ks], S[D - 2][(ks + 1) % 3],
+                                          S[D - 2][(ks + 2) % 3],
                                           q1 - 2 * (D - 1), acc, eo);
 #pragma unroll
         for (int s = D - 1; s >= 2; s--)
           if (active(s, t))
-            p.template stage<true>(S[s - 2][k], S[s - 2][(k + 1) % 3],
-                                   S[s - 2][(k + 2) % 3], q1 - 2 * (s - 1),
-                                   S[s - 1][k]);
+            p.template stage<true>(S[s - 2][ks], S[s - 2][(ks + 1) % 3],
+                                   S[s - 2][(ks + 2) % 3], q1 - 2 * (s - 1),
+                                   S[s - 1][ks]);
         if (active(1, t)) {
-          p.template stage<true>(R[k], R[(k + 1) % 3], R[(k + 2) % 3], q1,
-                                 S[0][k]);
-          if (q1 + 2 <= last_in) p.load(R[k], q1 + 2);
+          p.template stage<true>(R[k % RIN], R[(k + 1) % RIN],
+                                 R[(k + 2) % RIN], q1, S[0][ks]);
+          if (q1 - 1 + RIN <= last_in) p.load(R[k % RIN], q1 - 1 + RIN);
         }
       }
     }
   };
   int t0 = 0;
   for (;;) {
-    for (; t0 < n_steps && !(t0 >= tA && t0 < tB); t0 += 3) guarded(t0);
+    for (; t0 < n_steps && !(t0 >= tA && t0 < tB); t0 += U) guarded(t0);
     if (t0 >= n_steps) break;
-    for (; t0 < tB; t0 += 3) {
+    for (; t0 < tB; t0 += U) {
 #pragma unroll
-      for (int k = 0; k < 3; k++) {
+      for (int k = 0; k < U; k++) {
+        const int ks = k % 3;
         const int q1 = r0 - D + 1 + t0 + k;
-        p.template outrow<D, RES, PACK>(S[D - 2][k], S[D - 2][(k + 1) % 3],
-                                        S[D - 2][(k + 2) % 3],
+        p.template outrow<D, RES, PACK>(S[D - 2][ks], S[D - 2][(ks + 1) % 3],
+                                        S[D - 2][(ks + 2) % 3],
                                         q1 - 2 * (D - 1), acc, eo);
 #pragma unroll
-        for (int s = D - 1; s >= 2; s--) {
-          // groups of G stages: the compiler interleaves a group's
-          // independent stages, and a scheduling barrier keeps it from
-          // hoisting more (the registers of every stage in flight at once
-          // would not fit)
-          if ((D - s) % G == 0) __builtin_amdgcn_sched_barrier(0);
-          p.template stage<false>(S[s - 2][k], S[s - 2][(k + 1) % 3],
-                                  S[s - 2][(k + 2) % 3], q1 - 2 * (s - 1),
-                                  S[s - 1][k]);
-        }
-        __builtin_amdgcn_sched_barrier(0);
-        p.template stage<false>(R[k], R[(k + 1) % 3], R[(k + 2) % 3], q1,
-                                S[0][k]);
-        p.load(R[k], q1 + 2);
+        for (int s = D - 1; s >= 2; s--)
+          p.template stage<false>(S[s - 2][ks], S[s - 2][(ks + 1) % 3],
+                                  S[s - 2][(ks + 2) % 3], q1 - 2 * (s - 1),
+                                  S[s - 1][ks]);
+        p.template stage<false>(R[k % RIN], R[(k + 1) % RIN], R[(k + 2) % RIN],
+                                q1, S[0][ks]);
+        p.load(R[k % RIN], q1 - 1 + RIN);
       }
     }
   }

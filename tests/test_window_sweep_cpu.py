@@ -1,6 +1,6 @@
 """CPU model of the deep fused sweep's index logic (overlapping windows of
 64 lanes x 2 columns, D sweeps per pass, ring cells kept, guarded and steady
-steps, an input ring of 3 or 6 rows, leading copy stages), checked against
+steps, an input ring of 3 to 12 rows, leading copy stages), checked against
 D single sweeps of the oracle on chare blocks cut from a global grid.  The
 model restates the device code's loop and window arithmetic step by step,
 so a wrong window offset, ring column, store range, step split, input-ring
@@ -56,8 +56,8 @@ def sweep_window(D, inp, out, w, h, c0, r0, r1, rs, acc, eo, pack, rin=3,
                  n_copy=0):
     """sweep_panelD<D, RES, PACK, 0, rin> on one band of one window, the
     first n_copy stages copying."""
-    assert rin in (3, 6)
-    U = 3 if rin == 3 else 6
+    assert rin in (3, 6, 9, 12)
+    U = rin
     H = window_margin(D)
     lane = np.arange(64)
     c = c0 - H + 2 * lane
@@ -160,10 +160,13 @@ def sweep_window(D, inp, out, w, h, c0, r0, r1, rs, acc, eo, pack, rin=3,
             t0 += U
 
 
-def sweep_window_skew(D, inp, out, w, h, c0, r0, r1, rs, acc, eo, pack):
-    """sweep_panelD_skew<D, RES, PACK> on one band of one window: sweep s
-    makes row q1 - 2(s-1) at step t from rows sweep s-1 made at steps t-3,
+def sweep_window_skew(D, inp, out, w, h, c0, r0, r1, rs, acc, eo, pack,
+                      rin=3):
+    """sweep_panelD_skew<D, RES, PACK, rin> on one band of one window: sweep
+    s makes row q1 - 2(s-1) at step t from rows sweep s-1 made at steps t-3,
     t-2, t-1; the stages of a step run from the last to the first."""
+    assert rin in (3, 6, 9, 12)
+    U = rin
     H = window_margin(D)
     lane = np.arange(64)
     c = c0 - H + 2 * lane
@@ -211,8 +214,8 @@ def sweep_window_skew(D, inp, out, w, h, c0, r0, r1, rs, acc, eo, pack):
                 if pack:
                     eo.record(row, col, v)
 
-    R = [np.zeros((64, 2)) for _ in range(3)]
-    for k in range(3):
+    R = [np.zeros((64, 2)) for _ in range(rin)]
+    for k in range(rin):
         if r0 - D + k <= last_in:
             load(R[k], r0 - D + k)
     S = [[np.zeros((64, 2)) for _ in range(3)] for _ in range(D - 1)]
@@ -222,11 +225,11 @@ def sweep_window_skew(D, inp, out, w, h, c0, r0, r1, rs, acc, eo, pack):
     def active(s, t):
         return 3 * (s - 1) <= t <= n_rows + 2 * D - 4 + s
 
-    tA = (3 * D - 3 + 2) // 3 * 3
-    t_hi = n_rows + 2 * D - 3
+    tA = (3 * D - 3 + U - 1) // U * U
+    t_hi = n_rows + 2 * D - rin
     if bottom:
         t_hi = min(t_hi, h - r0 + D)
-    tB = tA + int((t_hi - tA) / 3) * 3   # C's division truncates toward zero
+    tB = tA + int((t_hi - tA) / U) * U   # C's division truncates toward zero
     if ringcols or tB < tA:
         tB = tA
 
@@ -238,7 +241,7 @@ def sweep_window_skew(D, inp, out, w, h, c0, r0, r1, rs, acc, eo, pack):
             assert (s - 2, q + d) in made, (s, q + d, t)
 
     def step(t, guarded):
-        k = t % 3
+        k, kr = t % 3, t % rin
         q1 = r0 - D + 1 + t
         if not guarded:
             for s in range(1, D + 1):
@@ -258,27 +261,28 @@ def sweep_window_skew(D, inp, out, w, h, c0, r0, r1, rs, acc, eo, pack):
                                     S[s - 2][(k + 2) % 3], q, guarded)
                 made[(s - 1, q)] = t
         if not guarded or active(1, t):
-            S[0][k] = stage(R[k], R[(k + 1) % 3], R[(k + 2) % 3], q1, guarded)
+            S[0][k] = stage(R[kr], R[(kr + 1) % rin], R[(kr + 2) % rin], q1,
+                            guarded)
             made[(0, q1)] = t
             if guarded:
-                if q1 + 2 <= last_in:
-                    load(R[k], q1 + 2)
+                if q1 - 1 + rin <= last_in:
+                    load(R[kr], q1 - 1 + rin)
             else:
-                load(R[k], q1 + 2)
+                load(R[kr], q1 - 1 + rin)
 
     t0 = 0
     while True:
         while t0 < n_steps and not (tA <= t0 < tB):
-            for k in range(3):
+            for k in range(U):
                 if t0 + k < n_steps:
                     step(t0 + k, True)
-            t0 += 3
+            t0 += U
         if t0 >= n_steps:
             break
         while t0 < tB:
-            for k in range(3):
+            for k in range(U):
                 step(t0 + k, False)
-            t0 += 3
+            t0 += U
 
 
 class Edges:
@@ -322,7 +326,8 @@ def chare_sweep(D, inp, out, rs, waves=16, band=0, rin=3, n_copy=0,
             r0 = 1 + b * band
             if skew:
                 sweep_window_skew(D, inp, out, bw, bh, 1 + panel * OW, r0,
-                                  min(r0 + band, bh + 1), rs, acc, eo, True)
+                                  min(r0 + band, bh + 1), rs, acc, eo, True,
+                                  rin)
             else:
                 sweep_window(D, inp, out, bw, bh, 1 + panel * OW, r0,
                              min(r0 + band, bh + 1), rs, acc, eo, True, rin,
@@ -338,7 +343,7 @@ def chare_sweep(D, inp, out, rs, waves=16, band=0, rin=3, n_copy=0,
             n = min(bh - r, end - pos)
             if skew:
                 sweep_window_skew(D, inp, out, bw, bh, 1 + panel * OW, 1 + r,
-                                  1 + r + n, rs, acc, eo, True)
+                                  1 + r + n, rs, acc, eo, True, rin)
             else:
                 sweep_window(D, inp, out, bw, bh, 1 + panel * OW, 1 + r,
                              1 + r + n, rs, acc, eo, True, rin, n_copy)
@@ -361,7 +366,7 @@ def cut(G, W, H, row0, col0, bw, bh):
     return ch
 
 
-@pytest.mark.parametrize("rin", [3, 6])
+@pytest.mark.parametrize("rin", [3, 6, 9, 12])
 @pytest.mark.parametrize("band", [0, 7, 64])
 @pytest.mark.parametrize("D,n_copy", [(3, 0), (4, 0), (5, 0), (6, 0), (7, 0),
                                       (8, 0), (12, 0), (12, 3), (16, 0),
@@ -428,13 +433,15 @@ def test_model_catches_a_wrong_ring_column():
     assert not np.array_equal(out.a[ii, jj], ref[1:H + 1, 1:W + 1])
 
 
+@pytest.mark.parametrize("rin", [3, 6, 12])
 @pytest.mark.parametrize("band", [0, 7, 64])
 @pytest.mark.parametrize("D", [3, 4, 5, 8, 9, 12, 16])
 @pytest.mark.parametrize("W,H,cr,cc,which", [
     (8, 8, 1, 1, (0, 0)), (12, 30, 2, 2, (1, 0)),
     (300, 200, 3, 5, (0, 0)), (300, 200, 3, 5, (1, 2)), (300, 200, 3, 5, (2, 4)),
     (260, 70, 1, 1, (0, 0)), (129, 257, 2, 2, (1, 1)), (1000, 40, 1, 3, (0, 2))])
-def test_skewed_window_sweep_matches_oracle(D, W, H, cr, cc, which, band):
+def test_skewed_window_sweep_matches_oracle(D, W, H, cr, cc, which, band,
+                                            rin):
     G = O.init_grid(W, H, seed=W + 5 * H + D)
     ref = G.copy()
     for _ in range(D):
@@ -449,7 +456,7 @@ def test_skewed_window_sweep_matches_oracle(D, W, H, cr, cc, which, band):
     inp = cut(G, W, H, row0, col0, bw, bh)
     out = Chare(bw, bh)
     rs = (ci == 0, ci == cr - 1, cj == 0, cj == cc - 1)
-    acc, eo = chare_sweep(D, inp, out, rs, band=band, skew=True)
+    acc, eo = chare_sweep(D, inp, out, rs, band=band, skew=True, rin=rin)
     ii, jj = out.idx(np.arange(1, bh + 1)[:, None], np.arange(1, bw + 1)[None, :])
     want = ref[row0 + 1: row0 + bh + 1, col0 + 1: col0 + bw + 1]
     assert np.array_equal(out.a[ii, jj], want)
