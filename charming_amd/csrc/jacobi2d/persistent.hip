@@ -327,27 +327,6 @@ __device__ inline T* wuni(T* p) {
 #define CHM_JACOBI_IN_RING 6
 #endif
 constexpr int kInRing = CHM_JACOBI_IN_RING;
-// 1: the deep pass runs on the skewed wavefront (sweep_panelD_skew: the D
-// stages of a step are independent); 0: the chained pass (sweep_panelD).
-#ifndef CHM_JACOBI_SKEW
-#define CHM_JACOBI_SKEW 0
-#endif
-constexpr bool kSkew = CHM_JACOBI_SKEW != 0;
-
-
-// One band of one window of the deep pass, in the selected schedule.
-template <int D, bool RES>
-__device__ __forceinline__ void deep_window(const double* in, double* out, int64_t P,
-                                   int bw, int bh, int c0, int r0, int r1,
-                                   u128acc& acc, const EdgeOut2& none,
-                                   RingSides rs) {
-  if (kSkew)
-    sweep_panelD_skew<D, RES, false, kInRing>(in, out, P, bw, bh, c0, r0, r1,
-                                              acc, none, rs);
-  else
-    sweep_panelD<D, RES, false, 0, kInRing>(in, out, P, bw, bh, c0, r0, r1,
-                                            acc, none, rs);
-}
 
 // D >= 3 fused sweeps of the whole block (halos D deep), overlapping
 // windows x bands.  The outgoing edges are packed afterwards (pack_edges):
@@ -388,8 +367,9 @@ __device__ __noinline__ void chare_sweepD(const double* in, double* out,
     for (int t = wave; t < S * nb; t += WV) {
       const int panel = t % S, b = t / S;
       const int r0 = 1 + b * band, r1 = min(r0 + band, bh + 1);
-      deep_window<D, RES>(in, out, P, bw, bh, 1 + panel * OW, r0, r1, acc,
-                          none, rs);
+      sweep_panelD<D, RES, false, 0, kInRing>(in, out, P, bw, bh,
+                                              1 + panel * OW, r0, r1, acc,
+                                              none, rs);
     }
   } else {
     // the S windows x bh rows, window after window, cut into one contiguous
@@ -401,8 +381,9 @@ __device__ __noinline__ void chare_sweepD(const double* in, double* out,
     while (pos < end) {
       const int panel = (int)(pos / bh), r = (int)(pos % bh);
       const int len = (int)min((int64_t)(bh - r), end - pos);
-      deep_window<D, RES>(in, out, P, bw, bh, 1 + panel * OW, 1 + r,
-                          1 + r + len, acc, none, rs);
+      sweep_panelD<D, RES, false, 0, kInRing>(in, out, P, bw, bh,
+                                              1 + panel * OW, 1 + r,
+                                              1 + r + len, acc, none, rs);
       pos += len;
     }
   }

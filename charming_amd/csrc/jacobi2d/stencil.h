@@ -750,8 +750,10 @@ __device__ inline void sweep_panelD(const double* __restrict__ in,
   // of the grid's ring is met (the ring rows 0 and h+1 only occur among the
   // first 2D-2 and the last D-1 steps) and no load runs past the band, so
   // those steps run without guards or ring checks unless the wave holds a
-  // ring column.  Both bounds are multiples of U, as the ring slots
-  // require.
+  // ring column.  (A second copy of the steady loop with ring-column
+  // selects for those waves raised the register pressure of the pass
+  // enough to spill in every step: 571 ms instead of 124 per 16384^2
+  // solve.)  Both bounds are multiples of U, as the ring slots require.
   const int tA = (2 * D - 2 + U - 1) / U * U;
   const int t_hi = min(n_steps - (D - 1), n_steps + 2 - RIN);
   int tB = tA + (t_hi - tA) / U * U;
@@ -809,109 +811,6 @@ __device__ inline void sweep_panelD(const double* __restrict__ in,
         p.template outrow<D, RES, PACK, RALL == 2>(
             S[D - 2][(ks + 1) % 3], S[D - 2][(ks + 2) % 3], S[D - 2][ks],
             q1 - (D - 1), acc, eo, RALL == 2 ? accd + (D - 1) : nullptr);
-      }
-    }
-  }
-}
-
-// The same D fused sweeps on a skewed wavefront: at step t, sweep s makes
-// row q1 - 2(s-1) (q1 = r0-D+1+t, the row the first sweep makes) from the
-// rows sweep s-1 made at steps t-3, t-2, t-1, never from one made at step
-// t.  So the D stages of a step are independent of each other (the
-// unskewed pass chains them: stage s needs the row stage s-1 has just
-// made), and a wave has D stages of work to interleave instead of one
-// chain.  The stages of a step run from the last to the first: stage s
-// reads stage s-1's slot t % 3 (its row of step t-3) before stage s-1
-// overwrites that slot with its row of step t, so each intermediate grid
-// still keeps three rows.  The pipeline is 3(D-1) steps deep instead of
-// 2(D-1): r1 - r0 + 3D - 3 steps per band.  Same arithmetic in the same
-// order as D single sweeps, so the result is bit-identical.
-template <int D, bool RES, bool PACK, int RIN = 3>
-__device__ inline void sweep_panelD_skew(const double* __restrict__ in,
-                                         double* __restrict__ out,
-                                         int64_t pitch, int w, int h, int c0,
-                                         int r0, int r1, u128acc& acc,
-                                         const EdgeOut2& eo, RingSides rs,
-                                         int cend = -1) {
-  static_assert(D >= 3 && D <= 16, "deep fused sweep: 3 <= D <= 16");
-  static_assert(RIN >= 3 && RIN <= 12 && RIN % 3 == 0,
-                "input ring of 3, 6, 9 or 12 rows");
-  constexpr int U = RIN;  // lcm(3, RIN)
-  if (r0 >= r1) return;
-  PanelW p;
-  p.setup(in, out, pitch, w, h, c0, cend < 0 ? w : cend, rs, D);
-  RowW R[RIN];        // input rows: row r0-D+i in slot i % RIN
-  RowW S[D - 1][3];   // S[s-1]: rows of the grid after s sweeps, made at
-                      // step t in slot t % 3
-  const int last_in = r1 + D - 1;  // the last input row of the band
-#pragma unroll
-  for (int k = 0; k < RIN; k++) {
-    R[k].v = make_double2(0.0, 0.0);
-    if (r0 - D + k <= last_in) p.load(R[k], r0 - D + k);
-  }
-  const int n_rows = r1 - r0;
-  const int n_steps = n_rows + 3 * D - 3;
-  // sweep s makes its rows at steps 3(s-1) .. n_rows + 2D - 4 + s (rows
-  // r0-D+s .. r1+D-1-s), the last sweep (stored) at 3(D-1) .. n_steps-1
-  auto active = [&](int s, int t) {
-    return t >= 3 * (s - 1) && t <= n_rows + 2 * D - 4 + s;
-  };
-  // Steps [tA, tB) form the steady state: every sweep is under way, the
-  // loads stay within the band and no row of the grid's ring is made (the
-  // top ring row only among the first 3D-3 steps; the bottom one, made
-  // first by sweep 1 at step h - r0 + D, bounds tB), so those steps run
-  // without guards or ring checks unless the wave holds a ring column.
-  // Both bounds are multiples of U, as the ring slots require.
-  const int tA = (3 * D - 3 + U - 1) / U * U;
-  int t_hi = n_rows + 2 * D - RIN;  // the last load stays in the band
-  if (rs.bottom) t_hi = min(t_hi, h - r0 + D);
-  int tB = tA + (t_hi - tA) / U * U;
-  if (p.ringcols || tB < tA) tB = tA;
-  auto guarded = [&](int t0) {
-#pragma unroll
-    for (int k = 0; k < U; k++) {
-      const int t = t0 + k;
-      const int ks = k % 3;
-      if (t < n_steps) {
-        const int q1 = r0 - D + 1 + t;
-        if (t >= 3 * (D - 1))
-          p.template outrow<D, RES, PACK>(S[D - 2][ks], S[D - 2][(ks + 1) % 3],
-                                          S[D - 2][(ks + 2) % 3],
-                                          q1 - 2 * (D - 1), acc, eo);
-#pragma unroll
-        for (int s = D - 1; s >= 2; s--)
-          if (active(s, t))
-            p.template stage<true>(S[s - 2][ks], S[s - 2][(ks + 1) % 3],
-                                   S[s - 2][(ks + 2) % 3], q1 - 2 * (s - 1),
-                                   S[s - 1][ks]);
-        if (active(1, t)) {
-          p.template stage<true>(R[k % RIN], R[(k + 1) % RIN],
-                                 R[(k + 2) % RIN], q1, S[0][ks]);
-          if (q1 - 1 + RIN <= last_in) p.load(R[k % RIN], q1 - 1 + RIN);
-        }
-      }
-    }
-  };
-  int t0 = 0;
-  for (;;) {
-    for (; t0 < n_steps && !(t0 >= tA && t0 < tB); t0 += U) guarded(t0);
-    if (t0 >= n_steps) break;
-    for (; t0 < tB; t0 += U) {
-#pragma unroll
-      for (int k = 0; k < U; k++) {
-        const int ks = k % 3;
-        const int q1 = r0 - D + 1 + t0 + k;
-        p.template outrow<D, RES, PACK>(S[D - 2][ks], S[D - 2][(ks + 1) % 3],
-                                        S[D - 2][(ks + 2) % 3],
-                                        q1 - 2 * (D - 1), acc, eo);
-#pragma unroll
-        for (int s = D - 1; s >= 2; s--)
-          p.template stage<false>(S[s - 2][ks], S[s - 2][(ks + 1) % 3],
-                                  S[s - 2][(ks + 2) % 3], q1 - 2 * (s - 1),
-                                  S[s - 1][ks]);
-        p.template stage<false>(R[k % RIN], R[(k + 1) % RIN], R[(k + 2) % RIN],
-                                q1, S[0][ks]);
-        p.load(R[k % RIN], q1 - 1 + RIN);
       }
     }
   }

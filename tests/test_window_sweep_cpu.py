@@ -160,131 +160,6 @@ def sweep_window(D, inp, out, w, h, c0, r0, r1, rs, acc, eo, pack, rin=3,
             t0 += U
 
 
-def sweep_window_skew(D, inp, out, w, h, c0, r0, r1, rs, acc, eo, pack,
-                      rin=3):
-    """sweep_panelD_skew<D, RES, PACK, rin> on one band of one window: sweep
-    s makes row q1 - 2(s-1) at step t from rows sweep s-1 made at steps t-3,
-    t-2, t-1; the stages of a step run from the last to the first."""
-    assert rin in (3, 6, 9, 12)
-    U = rin
-    H = window_margin(D)
-    lane = np.arange(64)
-    c = c0 - H + 2 * lane
-    vload = c <= w + D
-    vstore = (lane >= H // 2) & (lane < 64 - H // 2)
-    top, bottom, left, right = rs
-    ring_a = (left & (c == 0)) | (right & (c == w + 1))
-    ring_b = (left & (c + 1 == 0)) | (right & (c + 1 == w + 1))
-    ringcols = bool(np.any(ring_a | ring_b))
-    cend = w
-    last_in = r1 + D - 1
-    made = {}   # (sweep, row) -> step, to check what each stage reads
-
-    def load(r, row):
-        assert r0 - D <= row <= last_in
-        assert 1 - K_HALO <= row <= inp.bh + K_HALO
-        i, j = inp.idx(row, c[vload])
-        r[vload, 0] = inp.a[i, j]
-        r[vload, 1] = inp.a[i, j + 1]
-
-    def stage(up, cu, dn, q, check):
-        if check and ((q == 0 and top) or (q == h + 1 and bottom)):
-            return cu.copy()
-        a, b = cu[:, 0], cu[:, 1]
-        na = ((((a + shr1z(b)) + b) + up[:, 0]) + dn[:, 0]) * 0.2
-        nb = ((((b + a) + shl1z(a)) + up[:, 1]) + dn[:, 1]) * 0.2
-        if check and ringcols:
-            na = np.where(ring_a, a, na)
-            nb = np.where(ring_b, b, nb)
-        return np.stack([na, nb], axis=1)
-
-    def outrow(up, cu, dn, row):
-        a, b = cu[:, 0], cu[:, 1]
-        na = ((((a + shr1z(b)) + b) + up[:, 0]) + dn[:, 0]) * 0.2
-        nb = ((((b + a) + shl1z(a)) + up[:, 1]) + dn[:, 1]) * 0.2
-        for l in np.nonzero(vstore)[0]:
-            cl = c[l]
-            cells = [(cl, na[l], a[l])] if cl <= cend else []
-            if cl + 1 <= cend:
-                cells.append((cl + 1, nb[l], b[l]))
-            for col, v, old in cells:
-                i, j = out.idx(row, col)
-                out.a[i, j] = v
-                acc.append((v, old))
-                if pack:
-                    eo.record(row, col, v)
-
-    R = [np.zeros((64, 2)) for _ in range(rin)]
-    for k in range(rin):
-        if r0 - D + k <= last_in:
-            load(R[k], r0 - D + k)
-    S = [[np.zeros((64, 2)) for _ in range(3)] for _ in range(D - 1)]
-    n_rows = r1 - r0
-    n_steps = n_rows + 3 * D - 3
-
-    def active(s, t):
-        return 3 * (s - 1) <= t <= n_rows + 2 * D - 4 + s
-
-    tA = (3 * D - 3 + U - 1) // U * U
-    t_hi = n_rows + 2 * D - rin
-    if bottom:
-        t_hi = min(t_hi, h - r0 + D)
-    tB = tA + int((t_hi - tA) / U) * U   # C's division truncates toward zero
-    if ringcols or tB < tA:
-        tB = tA
-
-    def reads(s, q, t):
-        # rows q-1, q, q+1 of sweep s-1 (key s-2), each made at an earlier
-        # step (rows of the input, s = 1, are loaded, not made)
-        for d in (-1, 0, 1):
-            assert made.get((s - 2, q + d), -1) < t, (s, q + d, t)
-            assert (s - 2, q + d) in made, (s, q + d, t)
-
-    def step(t, guarded):
-        k, kr = t % 3, t % rin
-        q1 = r0 - D + 1 + t
-        if not guarded:
-            for s in range(1, D + 1):
-                q = q1 - 2 * (s - 1)
-                # the steady state must never meet what it skips
-                assert active(s, t)
-                assert not ((q == 0 and top) or (q == h + 1 and bottom))
-        if t >= 3 * (D - 1):
-            reads(D, q1 - 2 * (D - 1), t)
-            outrow(S[D - 2][k], S[D - 2][(k + 1) % 3], S[D - 2][(k + 2) % 3],
-                   q1 - 2 * (D - 1))
-        for s in range(D - 1, 1, -1):
-            if not guarded or active(s, t):
-                q = q1 - 2 * (s - 1)
-                reads(s, q, t)
-                S[s - 1][k] = stage(S[s - 2][k], S[s - 2][(k + 1) % 3],
-                                    S[s - 2][(k + 2) % 3], q, guarded)
-                made[(s - 1, q)] = t
-        if not guarded or active(1, t):
-            S[0][k] = stage(R[kr], R[(kr + 1) % rin], R[(kr + 2) % rin], q1,
-                            guarded)
-            made[(0, q1)] = t
-            if guarded:
-                if q1 - 1 + rin <= last_in:
-                    load(R[kr], q1 - 1 + rin)
-            else:
-                load(R[kr], q1 - 1 + rin)
-
-    t0 = 0
-    while True:
-        while t0 < n_steps and not (tA <= t0 < tB):
-            for k in range(U):
-                if t0 + k < n_steps:
-                    step(t0 + k, True)
-            t0 += U
-        if t0 >= n_steps:
-            break
-        while t0 < tB:
-            for k in range(U):
-                step(t0 + k, False)
-            t0 += U
-
-
 class Edges:
     """Outgoing edges D deep, as store_pair<D, RES, PACK> writes them."""
 
@@ -307,8 +182,7 @@ class Edges:
             self.right[col - (w - D + 1), row - 1] = v
 
 
-def chare_sweep(D, inp, out, rs, waves=16, band=0, rin=3, n_copy=0,
-                skew=False):
+def chare_sweep(D, inp, out, rs, waves=8, band=0, rin=3, n_copy=0):
     """chare_sweepD: with band > 0, bands of `band` rows x windows taken
     round-robin; else the windows x rows, window after window, cut into one
     contiguous range per wave (0: bands of 256 rows for chares of 1024+
@@ -324,14 +198,9 @@ def chare_sweep(D, inp, out, rs, waves=16, band=0, rin=3, n_copy=0,
         for t in range(S * nb):
             panel, b = t % S, t // S
             r0 = 1 + b * band
-            if skew:
-                sweep_window_skew(D, inp, out, bw, bh, 1 + panel * OW, r0,
-                                  min(r0 + band, bh + 1), rs, acc, eo, True,
-                                  rin)
-            else:
-                sweep_window(D, inp, out, bw, bh, 1 + panel * OW, r0,
-                             min(r0 + band, bh + 1), rs, acc, eo, True, rin,
-                             n_copy)
+            sweep_window(D, inp, out, bw, bh, 1 + panel * OW, r0,
+                         min(r0 + band, bh + 1), rs, acc, eo, True, rin,
+                         n_copy)
         return acc, eo
     total = S * bh
     per = (total + waves - 1) // waves
@@ -341,12 +210,8 @@ def chare_sweep(D, inp, out, rs, waves=16, band=0, rin=3, n_copy=0,
         while pos < end:
             panel, r = pos // bh, pos % bh
             n = min(bh - r, end - pos)
-            if skew:
-                sweep_window_skew(D, inp, out, bw, bh, 1 + panel * OW, 1 + r,
-                                  1 + r + n, rs, acc, eo, True, rin)
-            else:
-                sweep_window(D, inp, out, bw, bh, 1 + panel * OW, 1 + r,
-                             1 + r + n, rs, acc, eo, True, rin, n_copy)
+            sweep_window(D, inp, out, bw, bh, 1 + panel * OW, 1 + r,
+                         1 + r + n, rs, acc, eo, True, rin, n_copy)
             pos += n
     return acc, eo
 
@@ -433,33 +298,3 @@ def test_model_catches_a_wrong_ring_column():
     assert not np.array_equal(out.a[ii, jj], ref[1:H + 1, 1:W + 1])
 
 
-@pytest.mark.parametrize("rin", [3, 6, 12])
-@pytest.mark.parametrize("band", [0, 7, 64])
-@pytest.mark.parametrize("D", [3, 4, 5, 8, 9, 12, 16])
-@pytest.mark.parametrize("W,H,cr,cc,which", [
-    (8, 8, 1, 1, (0, 0)), (12, 30, 2, 2, (1, 0)),
-    (300, 200, 3, 5, (0, 0)), (300, 200, 3, 5, (1, 2)), (300, 200, 3, 5, (2, 4)),
-    (260, 70, 1, 1, (0, 0)), (129, 257, 2, 2, (1, 1)), (1000, 40, 1, 3, (0, 2))])
-def test_skewed_window_sweep_matches_oracle(D, W, H, cr, cc, which, band,
-                                            rin):
-    G = O.init_grid(W, H, seed=W + 5 * H + D)
-    ref = G.copy()
-    for _ in range(D):
-        ref = O.step(ref)
-    ci, cj = which
-    bh_base, bw_base = H // cr, W // cc
-    row0, col0 = ci * bh_base, cj * bw_base
-    bh = bh_base if ci < cr - 1 else H - bh_base * (cr - 1)
-    bw = bw_base if cj < cc - 1 else W - bw_base * (cc - 1)
-    if min(bw, bh) < D:
-        pytest.skip("the runtime refuses chares thinner than the halo depth")
-    inp = cut(G, W, H, row0, col0, bw, bh)
-    out = Chare(bw, bh)
-    rs = (ci == 0, ci == cr - 1, cj == 0, cj == cc - 1)
-    acc, eo = chare_sweep(D, inp, out, rs, band=band, skew=True, rin=rin)
-    ii, jj = out.idx(np.arange(1, bh + 1)[:, None], np.arange(1, bw + 1)[None, :])
-    want = ref[row0 + 1: row0 + bh + 1, col0 + 1: col0 + bw + 1]
-    assert np.array_equal(out.a[ii, jj], want)
-    assert len(acc) == bw * bh
-    assert np.array_equal(eo.top, want[:D, :])
-    assert np.array_equal(eo.right, want[:, bw - D:].T)
