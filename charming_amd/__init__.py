@@ -137,6 +137,8 @@ def jacobi_lib():
         L.chm_jacobi2d_reset.argtypes = [vp]
         L.chm_jacobi2d_solve.argtypes = [vp, C.POINTER(Result)]
         L.chm_jacobi2d_read_grid.argtypes = [vp, C.c_void_p, C.c_size_t]
+        L.chm_jacobi2d_chare_phases.argtypes = [vp, C.c_void_p, C.c_size_t,
+                                                C.POINTER(C.c_int)]
         L.chm_jacobi2d_hash_region.argtypes = [vp, C.c_int, C.c_int, C.c_int,
                                                C.c_int, C.POINTER(C.c_uint64)]
         L.chm_jacobi2d_destroy.argtypes = [vp]
@@ -219,6 +221,16 @@ class Jacobi2D:
         keys = ("sweep", "send", "wait", "recv")
         return {"mean_ms": dict(zip(keys, out[:4].round(3).tolist())),
                 "max_ms": dict(zip(keys, out[4:].round(3).tolist()))}
+
+    def chare_phases(self):
+        """Per-chare ms of the last persistent solve: an (n_chares, 4) array
+        of sweep, send, wait, recv (chares in index order, row-major)."""
+        L, n = jacobi_lib(), C.c_int(0)
+        _check(L.chm_jacobi2d_chare_phases(self.h, None, 0, C.byref(n)))
+        out = np.zeros((n.value, 4), dtype=np.float64)
+        _check(L.chm_jacobi2d_chare_phases(self.h, out.ctypes.data, out.size,
+                                           C.byref(n)))
+        return out
 
     def close(self):
         if getattr(self, "h", None):

@@ -53,6 +53,8 @@ int persistent_read_grid(Handle* h, double* out);
 int persistent_hash_region(Handle* h, int r0, int r1, int c0, int c1,
                            uint64_t* hash);
 int persistent_phase_stats(Handle* h, double* out);
+int persistent_chare_phases(Handle* h, double* out);
+int persistent_chare_count(Handle* h, int* n);
 void persistent_destroy(Handle* h);
 
 }  // namespace jac

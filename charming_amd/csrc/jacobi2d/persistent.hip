@@ -1131,7 +1131,12 @@ int persistent_read_grid(Handle* h, double* out) {
   return CHM_OK;
 }
 
-int persistent_phase_stats(Handle* h, double* out) {
+int persistent_chare_count(Handle* h, int* n) {
+  *n = h->ps->n_chares;
+  return CHM_OK;
+}
+
+int persistent_chare_phases(Handle* h, double* out) {
   PersistentState* ps = h->ps;
   std::vector<uint64_t> ph((size_t)ps->n_chares * 4);
   CHM_TRY_HIP(hipMemcpy(ph.data(), ps->desc.phase, ph.size() * 8,
@@ -1143,10 +1148,18 @@ int persistent_phase_stats(Handle* h, double* out) {
   CHM_TRY_HIP(hipMemcpy(rt.data(), ps->desc.recv_ticks, rt.size() * 8,
                         hipMemcpyDeviceToHost));
   for (int c = 0; c < ps->n_chares; c++) ph[4 * c + 3] = rt[c];
+  for (size_t e = 0; e < ph.size(); e++) out[e] = (double)ph[e] * ms_per_tick;
+  return CHM_OK;
+}
+
+int persistent_phase_stats(Handle* h, double* out) {
+  PersistentState* ps = h->ps;
+  std::vector<double> per((size_t)ps->n_chares * 4);
+  CHM_TRY(persistent_chare_phases(h, per.data()));
   for (int k = 0; k < 4; k++) {
     double sum = 0, mx = 0;
     for (int c = 0; c < ps->n_chares; c++) {
-      const double v = (double)ph[4 * c + k] * ms_per_tick;
+      const double v = per[4 * c + k];
       sum += v;
       mx = std::max(mx, v);
     }

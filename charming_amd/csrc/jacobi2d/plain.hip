@@ -562,6 +562,22 @@ int chm_jacobi2d_phase_stats(void* handle, double* out8) {
   return persistent_phase_stats(h, out8);
 }
 
+int chm_jacobi2d_chare_phases(void* handle, double* out, size_t out_elems,
+                              int* n_chares) {
+  Handle* h = (Handle*)handle;
+  if (!h || !n_chares) return fail(CHM_E_INVALID, "null argument");
+  if (h->form != CHM_FORM_PERSISTENT || !h->ps)
+    return fail(CHM_E_UNSUPPORTED, "phase times exist for the persistent form");
+  int n = 0;
+  CHM_TRY(persistent_chare_count(h, &n));
+  *n_chares = n;
+  if (!out) return CHM_OK;
+  if (out_elems < (size_t)n * 4)
+    return fail(CHM_E_INVALID, "output holds %zu elements, need %d", out_elems,
+                n * 4);
+  return persistent_chare_phases(h, out);
+}
+
 int chm_device_memory(size_t* free_bytes, size_t* total_bytes) {
   if (!free_bytes || !total_bytes) return fail(CHM_E_INVALID, "null argument");
   CHM_TRY_HIP(hipMemGetInfo(free_bytes, total_bytes));

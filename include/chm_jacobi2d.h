@@ -113,6 +113,14 @@ int chm_jacobi2d_hash_region(void* handle, int r0, int r1, int c0, int c1,
  * 148-166, 224-262). */
 int chm_jacobi2d_phase_stats(void* handle, double* out8);
 
+/* The same times per chare: out[4c .. 4c+3] = {sweep, send, wait-for-halos,
+ * inside halo-receive entry methods} of chare c, in ms (out_elems >= 4 x
+ * chares).  *n_chares receives the chare count; with out NULL only the count
+ * is written.  The reference prints these per chare (examples/jacobi2d/
+ * jacobi2d.cu:148-166). */
+int chm_jacobi2d_chare_phases(void* handle, double* out, size_t out_elems,
+                              int* n_chares);
+
 int chm_jacobi2d_destroy(void* handle);
 
 /* No counterpart in the reference.  For callers that cannot allocate device

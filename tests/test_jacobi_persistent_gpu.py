@@ -198,3 +198,26 @@ def test_thin_chares_fall_back_to_depth_one(cam):
         j.set_tuning(depth=3)
         with pytest.raises(cam.ChmError):
             j.solve()
+
+
+def test_phase_times_per_chare(cam):
+    # per-chare sweep/send/wait/recv times of the last solve, and their mean
+    # and maximum as phase_stats reports them; automatic split included
+    with cam.Jacobi2D(300, 200, 20, form="persistent", seed=3,
+                      chares=(3, 5)) as j:
+        j.solve()
+        ph = j.chare_phases()
+        st = j.phase_stats()
+        assert ph.shape == (15, 4)
+        assert (ph >= 0).all() and (ph[:, 0] > 0).all()
+        keys = list(st["mean_ms"])
+        for k, key in enumerate(keys):
+            assert st["mean_ms"][key] == pytest.approx(ph[:, k].mean(), abs=2e-3)
+            assert st["max_ms"][key] == pytest.approx(ph[:, k].max(), abs=2e-3)
+    with cam.Jacobi2D(256, 256, 4, form="persistent", seed=3) as j:
+        j.solve()
+        assert j.chare_phases().shape[1] == 4 and len(j.chare_phases()) >= 1
+    with cam.Jacobi2D(64, 64, 4, form="plain", seed=3) as j:
+        j.solve()
+        with pytest.raises(cam.ChmError):
+            j.chare_phases()
