@@ -253,12 +253,13 @@ __device__ void sweep_dispatch(const double* in, double* out, int64_t P,
                                int bw, int bh, bool res, const EdgeOut& eo,
                                int phase) {
   int nc = g_jdesc.tune_nc, u = g_jdesc.tune_u;
-  // Entry methods are reached through a function pointer, so their code is
-  // limited to 128 VGPRs whatever the launch bounds: only tiles whose row
-  // ring fits (NC * U <= 4) are instantiated; with 16 waves per PE that is
-  // 16 x 4 KiB of reads in flight per CU.
+  // The sweep runs from the work hook (256 VGPRs at 512-thread PEs).  Default
+  // 2x2: two 128-column chunks, two rows in flight per wave, 8 waves x 8 KiB
+  // of reads in flight per CU (16384^2 x 1000 at one sweep per exchange:
+  // 1x1 1157 ms, 1x2 1028, 1x4 959, 2x1 947, 2x2 914, 4x1 911-920, 4x2 923,
+  // 2x4 930, 4x4 933, 8x1 966, 8x2 979; the wider shapes spill).
   if (nc == 0) {
-    nc = 1;
+    nc = 2;
     u = 2;
   }
   if (u == 0) u = (4 / nc) > 0 ? 4 / nc : 1;
@@ -271,7 +272,8 @@ __device__ void sweep_dispatch(const double* in, double* out, int64_t P,
     return;                                                             \
   }
   CHM_CASE(1, 2) CHM_CASE(1, 4) CHM_CASE(2, 2) CHM_CASE(1, 1) CHM_CASE(2, 1)
-  CHM_CASE(4, 1)
+  CHM_CASE(4, 1) CHM_CASE(4, 2) CHM_CASE(2, 4) CHM_CASE(8, 1) CHM_CASE(4, 4)
+  CHM_CASE(8, 2)
 #undef CHM_CASE
   if (res)
     chare_sweep<1, 2, true>(in, out, P, bw, bh, eo, phase);

@@ -72,7 +72,9 @@ def test_reference_init_kat(cam):
 
 
 @pytest.mark.parametrize("nc,u,depth", [(1, 4, 0), (2, 2, 0), (4, 1, 1), (1, 1, 0),
-                                        (2, 1, 0), (1, 2, 1)])
+                                        (2, 1, 0), (1, 2, 1), (0, 0, 1),
+                                        (4, 2, 1), (2, 4, 1), (8, 1, 1),
+                                        (4, 4, 1), (8, 2, 1)])
 def test_tunings_and_rerun(cam, nc, u, depth):
     W, H, iters = 1100, 700, 8
     ref, res = O.jacobi_run(W, H, iters, seed=3)
