@@ -11,15 +11,18 @@ Prints ONE JSON line.  `value` is whole-job throughput in Gcell-upd/s
 (interior cells x iterations / wall seconds of the timed steps).
 
 Roofline of the dominant kernel (the persistent scheduler launch): the chares
-exchange halos D deep and fuse D sweeps per pass over their block (D = 8 by
-default), so the algorithm's compulsory HBM bytes are 16/D per cell-update
-(read u_k once, write u_{k+D} once per D updates).  `roofline.achieved` is
-that figure times the launch's cell-updates over its HIP-event duration;
-`roofline.traffic` is the bytes the memory system really moved (rocprof
-FETCH_SIZE + WRITE_SIZE, calibrated), beside its ratio to the minimum.  The
-16 B-per-update rate of the unfused sweep is reported separately
-(`gbs_at_16B_per_update`), and the one-sweep-per-exchange persistent form,
-which moves 16 B per update, is measured beside the plain forms.
+exchange halos D deep and fuse D sweeps per pass over their block (D = 16 by
+default for 1024^2 chares), so the algorithm's compulsory HBM bytes are 16/D
+per cell-update (read u_k once, write u_{k+D} once per D updates).
+`roofline.achieved` is that figure times the launch's cell-updates over its
+HIP-event duration; `roofline.traffic` is the bytes the memory system really
+moved (rocprof FETCH_SIZE + WRITE_SIZE, calibrated), beside its ratio to the
+minimum.  At sixteen fused sweeps the launch is bound by vector-ALU issue
+rather than by HBM (profiles/sq_valu_jacobi2d_16384_persistent.md), so the
+HBM fraction is not the binding one.  The 16 B-per-update rate of the
+unfused sweep is reported separately (`gbs_at_16B_per_update`), and the
+one-sweep-per-exchange persistent form, which moves 16 B per update, is
+measured beside the plain forms.
 
 Other blocks:
   "compare":  the plain per-launch and hipGraph forms of the same solve, the
