@@ -334,6 +334,50 @@ constexpr int kInRing = CHM_JACOBI_IN_RING;
 // windows x bands.  The outgoing edges are packed afterwards (pack_edges):
 // kept out of the sweep, they would cost its inner loop registers it needs
 // for the intermediate grids.
+// Which cells of the window whose output starts at column 1 + panel * OW
+// are ring columns of the grid (column 0 with rs.left, bw + 1 with
+// rs.right), by the lanes' own test in PanelW::setup over the window's 128
+// columns (lane l holds columns x0 + 2l and x0 + 2l + 1, x0 odd): 0 none,
+// 1 only lanes' first cells, 2 only their second cells, 3 both.
+template <int D>
+__host__ __device__ inline int ring_window(int panel, int bw, RingSides rs) {
+  const int x0 = 1 + panel * window_out_cols(D) - window_margin(D);
+  int k = 0;
+  if (rs.left && x0 <= 0 && x0 + 127 >= 0) k |= (0 - x0) % 2 ? 2 : 1;
+  if (rs.right && x0 <= bw + 1 && x0 + 127 >= bw + 1)
+    k |= (bw + 1 - x0) % 2 ? 2 : 1;
+  return k;
+}
+
+// This wave's tiles (window, band) of one kind: KIND = 1 or 2, the windows
+// whose ring columns fall on that cell of the lanes (their steady steps
+// keep them by selects, in a steady loop of their own, with a three-slot
+// input ring: with six the selects raise the register pressure of the
+// sixteen-deep pass enough to spill); KIND = 0, the others (a window with
+// ring columns on both cells runs every step guarded).  Waves take the
+// tiles round-robin in one order over all kinds, so neighbouring windows
+// (which read overlapping columns) run at the same rows at about the same
+// time.
+template <int D, bool RES, int KIND>
+__device__ __forceinline__ void band_tiles(const double* in, double* out,
+                                           int64_t P, int bw, int bh,
+                                           RingSides rs, int band,
+                                           u128acc& acc) {
+  const int wave = threadIdx.x / 64, WV = blockDim.x / 64;
+  constexpr int OW = window_out_cols(D);
+  const int S = (bw + OW - 1) / OW;
+  const int nb = (bh + band - 1) / band;
+  const EdgeOut2 none = {nullptr, nullptr, nullptr, nullptr};
+  for (int t = wave; t < S * nb; t += WV) {
+    const int panel = t % S, b = t / S;
+    const int k = ring_window<D>(panel, bw, rs);
+    if ((KIND == 0) ? (k == 1 || k == 2) : (k != KIND)) continue;
+    const int r0 = 1 + b * band, r1 = min(r0 + band, bh + 1);
+    sweep_panelD<D, RES, false, 0, KIND ? 3 : kInRing, KIND>(
+        in, out, P, bw, bh, 1 + panel * OW, r0, r1, acc, none, rs);
+  }
+}
+
 template <int D, bool RES>
 __device__ __noinline__ void chare_sweepD(const double* in, double* out,
                                           int64_t P, int bw, int bh,
@@ -362,17 +406,12 @@ __device__ __noinline__ void chare_sweepD(const double* in, double* out,
   int band = wuni(g_jdesc.tune_band);
   if (band == 0) band = bh >= 1024 ? 256 : -1;
   if (band > 0) {
-    // bands of `band` rows, windows of a band side by side: waves take the
-    // tiles round-robin, so neighbouring windows (which read overlapping
-    // columns) run at the same rows at about the same time
-    const int nb = (bh + band - 1) / band;
-    for (int t = wave; t < S * nb; t += WV) {
-      const int panel = t % S, b = t / S;
-      const int r0 = 1 + b * band, r1 = min(r0 + band, bh + 1);
-      sweep_panelD<D, RES, false, 0, kInRing>(in, out, P, bw, bh,
-                                              1 + panel * OW, r0, r1, acc,
-                                              none, rs);
+    // bands of `band` rows, windows of a band side by side
+    if (rs.left || rs.right) {
+      band_tiles<D, RES, 1>(in, out, P, bw, bh, rs, band, acc);
+      band_tiles<D, RES, 2>(in, out, P, bw, bh, rs, band, acc);
     }
+    band_tiles<D, RES, 0>(in, out, P, bw, bh, rs, band, acc);
   } else {
     // the S windows x bh rows, window after window, cut into one contiguous
     // range per wave: equal work whatever S is, at most a few bands each

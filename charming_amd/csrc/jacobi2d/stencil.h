@@ -637,12 +637,14 @@ struct PanelW {
 
   // One sweep: row q of the next grid from three rows of the current one
   // (COPY: the row itself, a stage that does no sweep).  CHECK = false is
-  // for rows and waves that meet no ring cell.
+  // for rows that meet no ring row; with it, COLS keeps the ring columns by
+  // selects without a branch (waves whose window holds one): bit 0 those
+  // among the lanes' first cells, bit 1 among their second cells.
   // RS = 1: add the sweep's exact residual over the cells this window
   // stores on rows [q0, q1) to *acc; RS = 2: add the plain FP64 sum of
   // (u' - u)^2 over them to *accd (a bound for the stop filter, not a
   // result).
-  template <bool CHECK, int RS = 0>
+  template <bool CHECK, int RS = 0, int COLS = 0>
   __device__ inline void stage(const RowW& up, const RowW& cu, const RowW& dn,
                                int q, RowW& o, bool copy = false,
                                u128acc* acc = nullptr, double* accd = nullptr,
@@ -661,6 +663,9 @@ struct PanelW {
     if (CHECK && ringcols) {
       if (ring_a) na = a;
       if (ring_b) nb = b;
+    } else if (!CHECK) {
+      if (COLS & 1) na = ring_a ? a : na;
+      if (COLS & 2) nb = ring_b ? b : nb;
     }
     o.v = make_double2(na, nb);
     if (RS != 0 && q >= q0 && q < q1 && vstore) {
@@ -714,8 +719,13 @@ struct PanelW {
 // deep).  RALL = 1 (with RES): stage s = 1 .. D-1 also adds its sweep's
 // exact residual over the stored cells to accs[s-1] (acc is the last
 // stage's); RALL = 2: every stage s = 1 .. D adds the FP64 sum of its
-// squared changes over the stored cells to accd[s-1].
-template <int D, bool RES, bool PACK, int RALL = 0, int RIN = 3>
+// squared changes over the stored cells to accd[s-1].  RINGC != 0: the
+// window holds ring columns of the grid only among its lanes' first cells
+// (1) or only among their second cells (2), and its steady steps keep them
+// by selects; otherwise a window with a ring column runs every step
+// guarded.
+template <int D, bool RES, bool PACK, int RALL = 0, int RIN = 3,
+          int RINGC = 0>
 __device__ inline void sweep_panelD(const double* __restrict__ in,
                                     double* __restrict__ out, int64_t pitch,
                                     int w, int h, int c0, int r0, int r1,
@@ -749,15 +759,16 @@ __device__ inline void sweep_panelD(const double* __restrict__ in,
   // Steps [tA, tB) form the steady state: every sweep is under way, no row
   // of the grid's ring is met (the ring rows 0 and h+1 only occur among the
   // first 2D-2 and the last D-1 steps) and no load runs past the band, so
-  // those steps run without guards or ring checks unless the wave holds a
-  // ring column.  (A second copy of the steady loop with ring-column
-  // selects for those waves raised the register pressure of the pass
-  // enough to spill in every step: 571 ms instead of 124 per 16384^2
-  // solve.)  Both bounds are multiples of U, as the ring slots require.
+  // those steps run without guards or ring checks; a wave that holds a
+  // ring column runs them with that column's selects (RINGC, its own
+  // instantiation: a select test inside one shared steady loop raised the
+  // register pressure of the pass enough to spill in every step, 571 ms
+  // instead of 124 per 16384^2 solve) or else guarded.  Both bounds are
+  // multiples of U, as the ring slots require.
   const int tA = (2 * D - 2 + U - 1) / U * U;
   const int t_hi = min(n_steps - (D - 1), n_steps + 2 - RIN);
   int tB = tA + (t_hi - tA) / U * U;
-  if (p.ringcols || tB < tA) tB = tA;
+  if ((!RINGC && p.ringcols) || tB < tA) tB = tA;
   auto guarded = [&](int t0) {
 #pragma unroll
     for (int k = 0; k < U; k++) {
@@ -797,13 +808,13 @@ __device__ inline void sweep_panelD(const double* __restrict__ in,
       for (int k = 0; k < U; k++) {
         const int ks = k % 3;
         const int q1 = r0 - D + 1 + t0 + k;
-        p.template stage<false, RALL>(R[k % RIN], R[(k + 1) % RIN],
-                                      R[(k + 2) % RIN], q1, S[0][ks],
-                                      n_copy >= 1, accs, accd, r0, r1);
+        p.template stage<false, RALL, RINGC>(R[k % RIN], R[(k + 1) % RIN],
+                                             R[(k + 2) % RIN], q1, S[0][ks],
+                                             n_copy >= 1, accs, accd, r0, r1);
         p.load(R[k % RIN], q1 - 1 + RIN);
 #pragma unroll
         for (int s = 2; s < D; s++)
-          p.template stage<false, RALL>(
+          p.template stage<false, RALL, RINGC>(
               S[s - 2][(ks + 1) % 3], S[s - 2][(ks + 2) % 3], S[s - 2][ks],
               q1 - (s - 1), S[s - 1][ks], n_copy >= s,
               RALL == 1 ? accs + (s - 1) : nullptr,

@@ -169,6 +169,21 @@ def test_deep_fused_sweeps_reference_init(cam, depth, iters, band):
         assert np.array_equal(j.grid(), ref) and r.residual == res
 
 
+@pytest.mark.parametrize("depth,band", [(3, 64), (8, 40), (9, 256), (16, 100)])
+@pytest.mark.parametrize("W", [1000, 1001])
+def test_deep_fused_ring_windows_both_cells(cam, depth, band, W):
+    # banded passes whose ring-column windows run their steady steps with
+    # selects: the right ring falls on a lane's second cell for odd chare
+    # widths and on its first for even ones
+    H, iters = 700, 2 * depth + 1
+    ref, res = O.jacobi_run(W, H, iters, seed=29)
+    with cam.Jacobi2D(W, H, iters, form="persistent", seed=29,
+                      chares=(2, 2)) as j:
+        j.set_tuning(1, 1, band_rows=band, depth=depth)
+        r = j.solve()
+        assert np.array_equal(j.grid(), ref) and r.residual == res
+
+
 def test_unsupported_fused_shape_is_refused(cam):
     with cam.Jacobi2D(256, 256, 4, form="persistent", seed=1, chares=(2, 2)) as j:
         j.set_tuning(4, 1, depth=2)

@@ -12,6 +12,7 @@ from oracle import oracle as O
 
 K_COL_OFF = 15   # element offset of halo column 0 within a row
 K_HALO = 16      # halo rows/columns every chare array holds
+PLANT_NO_SELECTS = False  # a planted mistake: steady steps drop ring selects
 
 
 def pitch_for_depth(w, d):
@@ -53,9 +54,11 @@ class Chare:
 
 
 def sweep_window(D, inp, out, w, h, c0, r0, r1, rs, acc, eo, pack, rin=3,
-                 n_copy=0):
-    """sweep_panelD<D, RES, PACK, 0, rin> on one band of one window, the
-    first n_copy stages copying."""
+                 n_copy=0, ringc=0):
+    """sweep_panelD<D, RES, PACK, 0, rin, ringc> on one band of one window,
+    the first n_copy stages copying; ringc = 1 or 2: the window's ring
+    columns fall only on the lanes' first or only on their second cells, and
+    the steady steps keep them by selects."""
     assert rin in (3, 6, 9, 12)
     U = rin
     H = window_margin(D)
@@ -67,6 +70,10 @@ def sweep_window(D, inp, out, w, h, c0, r0, r1, rs, acc, eo, pack, rin=3,
     ring_a = (left & (c == 0)) | (right & (c == w + 1))
     ring_b = (left & (c + 1 == 0)) | (right & (c + 1 == w + 1))
     ringcols = bool(np.any(ring_a | ring_b))
+    if ringc == 1:
+        assert not np.any(ring_b)
+    if ringc == 2:
+        assert not np.any(ring_a)
     cend = w
 
     def load(r, row):
@@ -77,7 +84,7 @@ def sweep_window(D, inp, out, w, h, c0, r0, r1, rs, acc, eo, pack, rin=3,
         r[vload, 0] = inp.a[i, j]
         r[vload, 1] = inp.a[i, j + 1]
 
-    def stage(up, cu, dn, q, check, copy=False):
+    def stage(up, cu, dn, q, check, copy=False, cols=0):
         if copy or (check and ((q == 0 and top) or (q == h + 1 and bottom))):
             return cu.copy()
         a, b = cu[:, 0], cu[:, 1]
@@ -86,6 +93,11 @@ def sweep_window(D, inp, out, w, h, c0, r0, r1, rs, acc, eo, pack, rin=3,
         if check and ringcols:
             na = np.where(ring_a, a, na)
             nb = np.where(ring_b, b, nb)
+        elif not check and not PLANT_NO_SELECTS:
+            if cols & 1:
+                na = np.where(ring_a, a, na)
+            if cols & 2:
+                nb = np.where(ring_b, b, nb)
         return np.stack([na, nb], axis=1)
 
     def outrow(up, cu, dn, row):
@@ -115,7 +127,7 @@ def sweep_window(D, inp, out, w, h, c0, r0, r1, rs, acc, eo, pack, rin=3,
     t_hi = min(n_steps - (D - 1), n_steps + 2 - rin)
     x = t_hi - tA
     tB = tA + int(x / U) * U   # C's division truncates toward zero
-    if ringcols or tB < tA:
+    if (not ringc and ringcols) or tB < tA:
         tB = tA
     t0 = 0
     while True:
@@ -147,14 +159,14 @@ def sweep_window(D, inp, out, w, h, c0, r0, r1, rs, acc, eo, pack, rin=3,
                 # the steady state must never meet what it skips
                 assert not ((q1 == 0 and top) or (q1 == h + 1 and bottom))
                 S[0][ks] = stage(R[k % rin], R[(k + 1) % rin], R[(k + 2) % rin],
-                                 q1, False, n_copy >= 1)
+                                 q1, False, n_copy >= 1, ringc)
                 load(R[k % rin], q1 - 1 + rin)
                 for s in range(2, D):
                     q = q1 - (s - 1)
                     assert not ((q == 0 and top) or (q == h + 1 and bottom))
                     S[s - 1][ks] = stage(S[s - 2][(ks + 1) % 3],
                                          S[s - 2][(ks + 2) % 3], S[s - 2][ks],
-                                         q, False, n_copy >= s)
+                                         q, False, n_copy >= s, ringc)
                 outrow(S[D - 2][(ks + 1) % 3], S[D - 2][(ks + 2) % 3],
                        S[D - 2][ks], q1 - (D - 1))
             t0 += U
@@ -182,9 +194,26 @@ class Edges:
             self.right[col - (w - D + 1), row - 1] = v
 
 
+def ring_window(D, panel, bw, rs):
+    """ring_window<D>: which cells of window `panel` are ring columns (0
+    none, 1 only the lanes' first cells, 2 only their second cells, 3
+    both)."""
+    top, bottom, left, right = rs
+    x0 = 1 + panel * window_out_cols(D) - window_margin(D)
+    k = 0
+    if left and x0 <= 0 <= x0 + 127:
+        k |= 2 if (0 - x0) % 2 else 1
+    if right and x0 <= bw + 1 <= x0 + 127:
+        k |= 2 if (bw + 1 - x0) % 2 else 1
+    return k
+
+
 def chare_sweep(D, inp, out, rs, waves=8, band=0, rin=3, n_copy=0):
     """chare_sweepD: with band > 0, bands of `band` rows x windows taken
-    round-robin; else the windows x rows, window after window, cut into one
+    round-robin, in three passes as band_tiles runs them (windows whose ring
+    columns fall on the lanes' first cells, then on their second cells, each
+    with a three-slot input ring and selects in the steady steps; then the
+    others); else the windows x rows, window after window, cut into one
     contiguous range per wave (0: bands of 256 rows for chares of 1024+
     rows)."""
     bw, bh = inp.bw, inp.bh
@@ -195,12 +224,17 @@ def chare_sweep(D, inp, out, rs, waves=8, band=0, rin=3, n_copy=0):
     if band > 0:
         acc, eo = [], Edges(bw, bh, D)
         nb = (bh + band - 1) // band
-        for t in range(S * nb):
-            panel, b = t % S, t // S
-            r0 = 1 + b * band
-            sweep_window(D, inp, out, bw, bh, 1 + panel * OW, r0,
-                         min(r0 + band, bh + 1), rs, acc, eo, True, rin,
-                         n_copy)
+        kinds = ([1, 2] if rs[2] or rs[3] else []) + [0]
+        for kind in kinds:
+            for t in range(S * nb):
+                panel, b = t % S, t // S
+                k = ring_window(D, panel, bw, rs)
+                if (k in (1, 2)) if kind == 0 else (k != kind):
+                    continue
+                r0 = 1 + b * band
+                sweep_window(D, inp, out, bw, bh, 1 + panel * OW, r0,
+                             min(r0 + band, bh + 1), rs, acc, eo, True,
+                             3 if kind else rin, n_copy, kind)
         return acc, eo
     total = S * bh
     per = (total + waves - 1) // waves
@@ -298,3 +332,34 @@ def test_model_catches_a_wrong_ring_column():
     assert not np.array_equal(out.a[ii, jj], ref[1:H + 1, 1:W + 1])
 
 
+
+
+@pytest.mark.parametrize("D", [3, 8, 16])
+def test_ring_windows_in_steady_steps(D):
+    # bands with steady steps on a chare holding both ring columns (even and
+    # odd widths put the right ring on either cell of a lane): the select
+    # passes reproduce the oracle, and dropping the selects is caught
+    global PLANT_NO_SELECTS
+    for W in (300, 301):
+        H = 90
+        G = O.init_grid(W, H, seed=W + D)
+        ref = G.copy()
+        for _ in range(D):
+            ref = O.step(ref)
+        want = ref[1:H + 1, 1:W + 1]
+        rs = (True, True, True, True)
+        kinds = {ring_window(D, p, W, rs)
+                 for p in range((W + window_out_cols(D) - 1) //
+                                window_out_cols(D))}
+        assert 2 in kinds and (1 in kinds) == (W % 2 == 0)
+        for plant in (False, True):
+            inp = cut(G, W, H, 0, 0, W, H)
+            out = Chare(W, H)
+            PLANT_NO_SELECTS = plant
+            try:
+                chare_sweep(D, inp, out, rs, band=40, rin=6)
+            finally:
+                PLANT_NO_SELECTS = False
+            ii, jj = out.idx(np.arange(1, H + 1)[:, None],
+                             np.arange(1, W + 1)[None, :])
+            assert np.array_equal(out.a[ii, jj], want) != plant
