@@ -333,7 +333,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--tune-iters", type=int, default=20,
+    ap.add_argument("--tune-iters", type=int, default=32,
                     help="sweeps per problem in the autotuner's samples")
     ap.add_argument("--serve-plan", default="10:100,50:250,100:1000,500:300,"
                     "2000:1000", help="rate:requests,... of the serving block")

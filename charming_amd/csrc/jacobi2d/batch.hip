@@ -163,14 +163,25 @@ __device__ void do_round2(const Request& r, const TileGeom& g, int src,
   if (RESB) add_residual(res_slot + 2, acc_b);
 }
 
-// D = 3 .. 8 sweeps of a tile from array `src` into array `dst` in one pass
-// (the deep pass of the chare program: overlapping 128-column windows, each
-// wave one contiguous share of the tile's windows x rows; kDeep - n_sw
-// leading stages copy).  RES: the last sweep's residual into res_slot[2..3].
+// D = 3 .. kDeep (or kDeepWide) sweeps of a tile from array `src` into
+// array `dst` in one pass (the deep pass of the chare program: overlapping
+// 128-column windows, each wave one contiguous share of the tile's windows
+// x rows).  FULL: all D sweeps (no stage copies, a compile-time fact);
+// otherwise D - n_sw leading stages copy.  RES: the last sweep's residual
+// into res_slot[2..3].  T (the engine's workgroup size) gives each engine
+// its own copy, compiled for that workgroup's register budget: sixteen
+// sweeps need the 256 registers of workgroups of 512 threads or fewer.
 // Kept out of line: its register rings would otherwise weigh on the
 // engine's task loop.
-constexpr int kDeep = 8;
-template <bool RES>
+constexpr int kDeep = 8;       // fixed sweep counts, 1024-thread workgroups
+constexpr int kDeepWide = 16;  // fixed sweep counts, up to 512 threads
+constexpr int kWideRing = 6;   // input-ring slots of the sixteen-deep pass
+
+__host__ __device__ constexpr int deep_for_threads(int threads) {
+  return threads <= 512 ? kDeepWide : kDeep;
+}
+
+template <int T, int D, bool RES, bool FULL>
 __device__ __noinline__ void do_round_deep(const double* in, double* out,
                                            int64_t pitch, int W, int H,
                                            int r0, int r1, int c0, int c1,
@@ -185,7 +196,7 @@ __device__ __noinline__ void do_round_deep(const double* in, double* out,
   c0 = uni(c0);
   c1 = uni(c1);
   n_sw = uni(n_sw);
-  constexpr int OW = jac::window_out_cols(kDeep);
+  constexpr int OW = jac::window_out_cols(D);
   const int rows = r1 - r0, S = (c1 - c0 + OW - 1) / OW;
   const int WV = blockDim.x / 64, wave = threadIdx.x / 64;
   const int64_t total = (int64_t)S * rows;
@@ -198,10 +209,9 @@ __device__ __noinline__ void do_round_deep(const double* in, double* out,
   while (pos < end) {
     const int panel = (int)(pos / rows), rr = (int)(pos % rows);
     const int len = (int)min((int64_t)(rows - rr), end - pos);
-    jac::sweep_panelD<kDeep, RES, false>(in, out, pitch, W, H,
-                                         c0 + panel * OW, r0 + rr,
-                                         r0 + rr + len, acc, none, rs, c1 - 1,
-                                         kDeep - n_sw);
+    jac::sweep_panelD<D, RES, false, 0, D == kDeepWide ? kWideRing : 3>(
+        in, out, pitch, W, H, c0 + panel * OW, r0 + rr, r0 + rr + len, acc,
+        none, rs, c1 - 1, FULL ? 0 : D - n_sw);
     pos += len;
   }
   if (RES) add_residual(res_slot + 2, acc);
@@ -621,7 +631,7 @@ __device__ int find_work(const EngineDesc& E, uint32_t* task, uint64_t* req,
 
 // Round rr of a tile: `depth` fused sweeps (or what is left in the last
 // round), with the residuals the problem needs.
-template <int NC, int U>
+template <int T, int NC, int U>
 __device__ void run_round(const Request& r, const TileGeom& g, int rr,
                           Prob& p) {
   const bool tol = r.tol2 >= 0;
@@ -636,12 +646,37 @@ __device__ void run_round(const Request& r, const TileGeom& g, int rr,
     do_round_tol<1>(arr(r, src), arr(r, dst), r.pitch, r.W, r.H, g.r0, g.r1,
                     g.c0, g.c1, n_sw, slot, nullptr);
   } else if (n_sw >= 3) {
-    if (last)
-      do_round_deep<true>(arr(r, src), arr(r, dst), r.pitch, r.W, r.H, g.r0,
-                          g.r1, g.c0, g.c1, n_sw, slot);
+    // every round but the last sweeps `depth` times; a shorter last round
+    // runs the shallowest pass that holds it (eight stages, or sixteen),
+    // its leading stages copying
+    const double* in = arr(r, src);
+    double* out = arr(r, dst);
+    if constexpr (deep_for_threads(T) == kDeepWide) {
+      if (n_sw > kDeep) {
+        if (!last)
+          do_round_deep<T, kDeepWide, false, true>(in, out, r.pitch, r.W,
+                                                   r.H, g.r0, g.r1, g.c0,
+                                                   g.c1, n_sw, slot);
+        else if (n_sw == kDeepWide)
+          do_round_deep<T, kDeepWide, true, true>(in, out, r.pitch, r.W, r.H,
+                                                  g.r0, g.r1, g.c0, g.c1,
+                                                  n_sw, slot);
+        else
+          do_round_deep<T, kDeepWide, true, false>(in, out, r.pitch, r.W,
+                                                   r.H, g.r0, g.r1, g.c0,
+                                                   g.c1, n_sw, slot);
+        return;
+      }
+    }
+    if (!last)
+      do_round_deep<T, kDeep, false, true>(in, out, r.pitch, r.W, r.H, g.r0,
+                                           g.r1, g.c0, g.c1, n_sw, slot);
+    else if (n_sw == kDeep)
+      do_round_deep<T, kDeep, true, true>(in, out, r.pitch, r.W, r.H, g.r0,
+                                          g.r1, g.c0, g.c1, n_sw, slot);
     else
-      do_round_deep<false>(arr(r, src), arr(r, dst), r.pitch, r.W, r.H, g.r0,
-                           g.r1, g.c0, g.c1, n_sw, slot);
+      do_round_deep<T, kDeep, true, false>(in, out, r.pitch, r.W, r.H, g.r0,
+                                           g.r1, g.c0, g.c1, n_sw, slot);
   } else if (n_sw == 1) {
     if (tol || last)
       do_sweep<NC, U, true>(r, g, src, dst, slot);
@@ -715,7 +750,7 @@ __global__ void __launch_bounds__(T) engine_kernel(EngineDesc E) {
     if (kind == K_INIT) {
       do_init(r, g);
     } else if (kind == K_ITER) {
-      run_round<NC, U>(r, g, k, p);
+      run_round<T, NC, U>(r, g, k, p);
     } else if (kind == K_FIX && uni(s_n) < 0) {
       // round k again from its intact input, now with exact residuals (the
       // grid it writes is bit-identical to what the round wrote)
@@ -898,14 +933,16 @@ static double now_s() {
 
 // Tile side for a problem: `block` (> 0), or a rule that keeps tiles large
 // enough to amortise a task's fences and small enough to spread the problem,
-// in multiples of the deep pass's 112-column window (measured on the
-// 512-problem batch: 116-118 ms, against 158 ms with sides 64/128/256/512).
-static int pick_block(int W, int H, int block) {
+// in multiples of the deep pass's window: 112 columns at eight sweeps
+// (measured on the 512-problem batch: 116-118 ms, against 158 ms with sides
+// 64/128/256/512), 96 at sixteen.
+static int pick_block(int W, int H, int block, int deep) {
   if (block > 0) return block;
   const int side = std::max(W, H);
-  if (side <= 256) return 112;
-  if (side <= 1024) return 224;
-  return 448;
+  const int ow = jac::window_out_cols(deep);
+  if (side <= 256) return ow;
+  if (side <= 1024) return 2 * ow;
+  return 4 * ow;
 }
 
 struct Built {
@@ -913,13 +950,16 @@ struct Built {
   size_t grid_bytes, pend_bytes;
 };
 
-static Built build_request(const chm_batch_problem& pb, int block) {
+// deep: the rounds of fixed sweep counts (kDeep, or kDeepWide for engines
+// of up to 512-thread workgroups).
+static Built build_request(const chm_batch_problem& pb, int block, int deep) {
   Built b;
   memset(&b.r, 0, sizeof(b.r));
   Request& r = b.r;
   r.W = pb.width;
   r.H = pb.height;
-  r.bs = pick_block(pb.width, pb.height, pb.block > 0 ? pb.block : block);
+  r.bs = pick_block(pb.width, pb.height, pb.block > 0 ? pb.block : block,
+                    deep);
   r.tr = (r.H + r.bs - 1) / r.bs;
   r.tc = (r.W + r.bs - 1) / r.bs;
   r.max_iters = pb.iters;
@@ -927,12 +967,17 @@ static Built build_request(const chm_batch_problem& pb, int block) {
   r.boundary = pb.boundary;
   r.tol2 = pb.tol > 0 ? pb.tol * pb.tol : -1.0;
   r.seed = pb.seed;
-  // rounds of eight fused sweeps for a fixed sweep count, six with a
-  // tolerance (the pass then sums every sweep's residual, in registers);
-  // tiles and grid at least that many cells on a side, else two
+  // rounds of `deep` (sixteen or eight) fused sweeps for a fixed sweep
+  // count, six with a tolerance (the pass then sums every sweep's residual,
+  // in registers); tiles and grid at least that many cells on a side, else
+  // eight, else two
   const int side = std::min(std::min(r.W, r.H), r.bs);
-  const int deep = r.tol2 < 0 ? kDeep : kDeepTol;
-  r.depth = side >= deep ? deep : 2;
+  if (r.tol2 >= 0)
+    r.depth = side >= kDeepTol ? kDeepTol : 2;
+  else if (side >= deep)
+    r.depth = deep;
+  else
+    r.depth = side >= kDeep ? kDeep : 2;
   r.rounds = (pb.iters + r.depth - 1) / r.depth;
   r.pitch = jac::pitch_for_depth(r.W, r.depth);  // halos `depth` deep
   b.grid_bytes = round_up(
@@ -941,14 +986,15 @@ static Built build_request(const chm_batch_problem& pb, int block) {
   return b;
 }
 
-static int check_problem(const chm_batch_problem& p, int block) {
+static int check_problem(const chm_batch_problem& p, int block, int deep) {
   if (p.width < 1 || p.height < 1 || p.iters < 1 || p.width > (1 << 20) ||
       p.height > (1 << 20))
     return fail(CHM_E_INVALID, "bad problem %dx%d iters %d", p.width,
                 p.height, p.iters);
   if (p.init_mode != CHM_INIT_SEEDED && p.init_mode != CHM_INIT_REFERENCE)
     return fail(CHM_E_INVALID, "bad init mode %d", p.init_mode);
-  const int bs = pick_block(p.width, p.height, p.block > 0 ? p.block : block);
+  const int bs =
+      pick_block(p.width, p.height, p.block > 0 ? p.block : block, deep);
   if (bs < 16 || bs > 8192)
     return fail(CHM_E_INVALID, "tile side %d (16..8192)", bs);
   const int64_t tiles =
@@ -1157,7 +1203,7 @@ static void store_out(const BatchOut& out, int i, const Completion& c) {
 // n_arr: 3 for the engine (rounds rotate over three arrays), 2 for the
 // plain form.
 static int layout_batch(const chm_batch_problem* probs, int n, int block,
-                        int iters_cap, const int* blocks, int n_arr,
+                        int iters_cap, const int* blocks, int n_arr, int deep,
                         std::vector<Built>& bl, size_t* total, uint64_t* tiles) {
   bl.resize(n);
   *total = 0;
@@ -1166,8 +1212,8 @@ static int layout_batch(const chm_batch_problem* probs, int n, int block,
     chm_batch_problem pb = probs[i];
     if (iters_cap > 0) pb.iters = std::min(pb.iters, iters_cap);
     if (blocks && blocks[i] > 0) pb.block = blocks[i];
-    CHM_TRY(check_problem(pb, block));
-    bl[i] = build_request(pb, block);
+    CHM_TRY(check_problem(pb, block, deep));
+    bl[i] = build_request(pb, block, deep);
     *total += n_arr * bl[i].grid_bytes + bl[i].pend_bytes;
     *tiles += (uint64_t)bl[i].r.tr * bl[i].r.tc;
   }
@@ -1203,15 +1249,18 @@ static int batch_once(const chm_batch_problem* probs, int n,
                       const int* blocks, BatchOut out) {
   if (n < 1 || n > (int)kMaxSlots)
     return fail(CHM_E_INVALID, "%d problems (1..%u)", n, kMaxSlots);
-  const int threads = cfg && cfg->wg_threads ? cfg->wg_threads : 1024;
+  // (default 512 threads: sixteen sweeps per round, 105-108 ms for the
+  // 512-problem batch at the size rule's tiles, against 123-124 ms at 1024
+  // threads and eight sweeps per round, and 136 ms at 256 threads)
+  const int threads = cfg && cfg->wg_threads ? cfg->wg_threads : 512;
   const int block = cfg ? cfg->block : 0;
   const int nc = cfg && cfg->panel_chunks ? cfg->panel_chunks : 1;
   const int u = cfg && cfg->rows_in_flight ? cfg->rows_in_flight : 2;
   std::vector<Built> bl;
   size_t total = 0;
   uint64_t tiles = 0;
-  CHM_TRY(layout_batch(probs, n, block, iters_cap, blocks, 3, bl, &total,
-                       &tiles));
+  CHM_TRY(layout_batch(probs, n, block, iters_cap, blocks, 3,
+                       deep_for_threads(threads), bl, &total, &tiles));
   Engine e;
   int rc = engine_setup(e, threads, nc, u, (uint32_t)n, tiles, false);
   char* arena = nullptr;
@@ -1568,7 +1617,8 @@ int chm_jacobi2d_batch_plain(const chm_batch_problem* probs, int n,
   std::vector<Built> bl;
   size_t total = 0;
   uint64_t tiles = 0;
-  CHM_TRY(layout_batch(probs, n, 0, 0, nullptr, 2, bl, &total, &tiles));
+  CHM_TRY(layout_batch(probs, n, 0, 0, nullptr, 2, kDeep, bl, &total,
+                       &tiles));
   PlainCtx c;
   int rc = plain_ctx_init(c);
   char* arena = nullptr;
@@ -1645,10 +1695,15 @@ int chm_jacobi2d_batch_autotune(const chm_batch_problem* probs, int n,
                                 int sample_iters, chm_batch_config* best,
                                 int* blocks_out, double* best_rate) {
   if (!probs || !best || n < 1) return fail(CHM_E_INVALID, "null argument");
-  const int cap = sample_iters > 0 ? sample_iters : 20;
-  static const int kBlocks[] = {64, 112, 128, 224, 256, 448, 512};
+  // whole rounds for every candidate: a sample that ends inside a round of
+  // sixteen pays for the round's copying stages and misranks the engines
+  const int cap = ((sample_iters > 0 ? sample_iters : 32) + 15) / 16 * 16;
+  // tile sides tried per size class: the powers of two and the multiples
+  // of the deep pass's window (112 columns at eight sweeps, 96 at sixteen)
+  static const int kBlocks8[] = {64, 112, 128, 224, 256, 448, 512};
+  static const int kBlocks16[] = {64, 96, 128, 192, 256, 384, 512};
   // size classes: one per octave of the longer side
-  std::vector<int> cls(n), blocks(n);
+  std::vector<int> cls(n);
   int n_cls = 0;
   for (int i = 0; i < n; i++) {
     const int side = std::max(probs[i].width, probs[i].height);
@@ -1656,8 +1711,15 @@ int chm_jacobi2d_batch_autotune(const chm_batch_problem* probs, int n,
     while ((1 << (c + 7)) < side) c++;  // <=128, <=256, ...
     cls[i] = c;
     n_cls = std::max(n_cls, c + 1);
-    blocks[i] = pick_block(probs[i].width, probs[i].height, 0);
   }
+  // the size rule's tiles for an engine of `threads`
+  auto rule_blocks = [&](int threads) {
+    std::vector<int> b(n);
+    for (int i = 0; i < n; i++)
+      b[i] = pick_block(probs[i].width, probs[i].height, 0,
+                        deep_for_threads(threads));
+    return b;
+  };
   auto rate_of = [&](const chm_batch_config& c, const std::vector<int>& b,
                      double* rate) {
     chm_batch_result r;
@@ -1685,17 +1747,23 @@ int chm_jacobi2d_batch_autotune(const chm_batch_problem* probs, int n,
       c.panel_chunks = sh[0];
       c.rows_in_flight = sh[1];
       double rate = 0;
-      if (rate_of(c, blocks, &rate) == CHM_OK && rate > top) {
+      if (rate_of(c, rule_blocks(t), &rate) == CHM_OK && rate > top) {
         top = rate;
         top_cfg = c;
       }
     }
   if (top < 0) return fail(CHM_E_DEVICE, "no autotune candidate ran");
+  std::vector<int> blocks = rule_blocks(top_cfg.wg_threads);
+  const bool wide = deep_for_threads(top_cfg.wg_threads) == kDeepWide;
+  const int* kBlocks = wide ? kBlocks16 : kBlocks8;
+  const int n_blocks = wide ? (int)(sizeof(kBlocks16) / sizeof(int))
+                            : (int)(sizeof(kBlocks8) / sizeof(int));
   // 2. tile side per size class, one class at a time (largest first)
   for (int c = n_cls - 1; c >= 0; c--) {
     std::vector<int> trial = blocks;
     int best_b = -1;
-    for (int b : kBlocks) {
+    for (int bi = 0; bi < n_blocks; bi++) {
+      const int b = kBlocks[bi];
       bool any = false;
       for (int i = 0; i < n; i++)
         if (cls[i] == c) {
@@ -1785,8 +1853,9 @@ int chm_jacobi2d_server_submit(void* handle, const chm_batch_problem* pb,
   int block = s->block;
   if (engine && pb->block <= 0 && block <= 0)
     block = serve_block(pb->width, pb->height, s->e.d.n_wg, (int)s->live.size());
-  CHM_TRY(check_problem(*pb, block));
-  Built b = build_request(*pb, block);
+  const int deep = engine ? deep_for_threads(s->e.threads) : kDeep;
+  CHM_TRY(check_problem(*pb, block, deep));
+  Built b = build_request(*pb, block, deep);
   const size_t bytes = (engine ? 3 : 2) * b.grid_bytes + (engine ? b.pend_bytes : 0);
   if (bytes > s->pool_size)
     return fail(CHM_E_INVALID, "request needs %zu bytes, budget %zu", bytes,

@@ -187,8 +187,10 @@ typedef struct chm_batch_problem {
 } chm_batch_problem;
 
 typedef struct chm_batch_config {
-  int wg_threads;     /* 256, 512 or 1024 (0 = 1024): up to 1024 / wg_threads
-                         workgroups per CU, as many as stay resident */
+  int wg_threads;     /* 256, 512 or 1024 (0 = 512): up to 1024 / wg_threads
+                         workgroups per CU, as many as stay resident; fixed
+                         sweep counts go in rounds of sixteen fused sweeps
+                         at 512 threads or fewer, of eight at 1024 */
   int block;          /* tile side for problems with block 0 (0 = by size) */
   int panel_chunks;   /* sweep shape: 1 or 2 (0 = 1) */
   int rows_in_flight; /* sweep shape: 2 or 4 (0 = 2) */
@@ -222,9 +224,12 @@ int chm_jacobi2d_batch_plain(const chm_batch_problem* probs, int n,
                              int* iters_done);
 
 /* Autotuner: picks the workgroup size and sweep shape, and a tile side per
- * problem from {64, 112, 128, 224, 256, 448, 512} (blocks_out[i]), by coordinate descent
- * over size classes, timing short engine runs of the whole batch capped at
- * sample_iters sweeps.  best_rate = cell updates per second of the best. */
+ * problem (blocks_out[i]) from {64, 96, 128, 192, 256, 384, 512} for
+ * workgroups of 512 threads or fewer, {64, 112, 128, 224, 256, 448, 512}
+ * for 1024, by coordinate descent over size classes, timing short engine
+ * runs of the whole batch with every problem capped at sample_iters sweeps
+ * rounded up to a multiple of 16 (0 = 32).  best_rate = cell updates per
+ * second of the best. */
 int chm_jacobi2d_batch_autotune(const chm_batch_problem* probs, int n,
                                 int sample_iters, chm_batch_config* best,
                                 int* blocks_out, double* best_rate);

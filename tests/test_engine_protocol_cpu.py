@@ -55,20 +55,25 @@ def sweep_region(src, r0, r1, c0, c1, H, W):
     return out
 
 
-def depth_for(W, H, bs, tol2):
-    """Sweeps per round (build_request): 8 for a fixed sweep count, 6 with a
-    tolerance, 2 when the tile or grid is thinner than that."""
-    deep = 8 if tol2 < 0 else 6
-    return deep if min(W, H, bs) >= deep else 2
+def depth_for(W, H, bs, tol2, deep=8):
+    """Sweeps per round (build_request): `deep` (16 for engines of up to
+    512-thread workgroups, else 8) for a fixed sweep count, 6 with a
+    tolerance; 8, then 2, when the tile or grid is thinner than that."""
+    side = min(W, H, bs)
+    if tol2 >= 0:
+        return 6 if side >= 6 else 2
+    if side >= deep:
+        return deep
+    return 8 if side >= 8 else 2
 
 
 class Problem:
-    def __init__(self, W, H, bs, iters, tol, seed):
+    def __init__(self, W, H, bs, iters, tol, seed, deep=8):
         self.W, self.H, self.bs, self.iters = W, H, bs, iters
         self.tol2 = tol * tol if tol > 0 else -1.0
         self.tr, self.tc = -(-H // bs), -(-W // bs)
         self.n = self.tr * self.tc
-        self.depth = depth_for(W, H, bs, self.tol2)
+        self.depth = depth_for(W, H, bs, self.tol2, deep)
         self.rounds = -(-iters // self.depth)
         g = O.init_grid(W, H, seed)
         self.A = [g.copy(), g.copy(), g.copy()]
@@ -390,6 +395,27 @@ def test_protocol_matches_oracle_under_random_interleavings(seed):
         assert p.stop_iters == n, (W, H, bs, it, tol)
         assert p.hash == O.grid_hash(g), (W, H, bs, it, tol)
         assert p.s_last == s_last, (W, H, bs, it, tol)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_protocol_with_sixteen_deep_rounds(seed):
+    # fixed sweep counts in rounds of sixteen (tiles and grids of 16 cells
+    # or more), eight or two below that, under random interleavings
+    cases = [(40, 30, 16, 9, 0.0), (33, 17, 16, 37, 0.0), (40, 40, 16, 29, 0.0),
+             (64, 48, 16, 50, 0.0), (20, 20, 8, 17, 0.0), (12, 12, 4, 5, 0.0),
+             (40, 40, 20, 16, 0.0), (50, 40, 16, 400, 3e-2)]
+    rng = random.Random(seed)
+    probs = [Problem(W, H, bs, it, tol, 700 * seed + i, deep=16)
+             for i, (W, H, bs, it, tol) in enumerate(cases)]
+    assert {p.depth for p in probs} == {16, 8, 6, 2}
+    Engine(probs, rng).run(max_overlap=12)
+    for i, ((W, H, bs, it, tol), p) in enumerate(zip(cases, probs)):
+        s = 700 * seed + i
+        if tol > 0:
+            g, n, s_last, s0 = O.jacobi_solve_tol(W, H, it, tol, seed=s)
+        else:
+            (g, s_last), n = O.jacobi_run(W, H, it, seed=s), it
+        assert (p.stop_iters, p.hash, p.s_last) == (n, O.grid_hash(g), s_last)
 
 
 def test_protocol_with_every_filtered_round_recomputed(monkeypatch):

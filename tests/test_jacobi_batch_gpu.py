@@ -65,6 +65,28 @@ def test_engine_shapes_bit_identical(cam, threads, nc, u):
     check_against_oracle(probs, out)
 
 
+@pytest.mark.parametrize("threads", [256, 512])
+def test_sixteen_deep_rounds_match_oracle(cam, threads):
+    # engines of up to 512-thread workgroups sweep fixed-count problems in
+    # rounds of sixteen: several full rounds, last rounds of 1..15 sweeps
+    # (two-sweep and copying passes), tiles of 16..384 and thinner grids
+    # that fall back to eight or two
+    rng = np.random.default_rng(40 + threads)
+    probs = []
+    for i in range(20):
+        probs.append({"width": int(rng.integers(12, 420)),
+                      "height": int(rng.integers(12, 420)),
+                      "iters": int(rng.integers(16, 90)),
+                      "block": int(rng.choice([16, 40, 96, 192, 384, 0])),
+                      "seed": 4000 + threads + i, "init_mode": int(i % 4 == 3),
+                      "boundary": 1.0})
+    probs += [{"width": 300, "height": 200, "iters": k, "seed": 4500 + k}
+              for k in (31, 41, 47, 48)]
+    print("sixteen-deep batch", threads, len(probs), flush=True)
+    out = cam.batch_run(probs, wg_threads=threads)
+    check_against_oracle(probs, out)
+
+
 def test_problem_alone_equals_in_mix(cam):
     probs = mixed_problems(30, 3, side_hi=400, it_hi=30)
     mix = cam.batch_run(probs)
@@ -183,7 +205,13 @@ def test_autotune_small_batch(cam):
     cfg, blocks, rate = cam.batch_autotune(probs, sample_iters=5)
     assert cfg["wg_threads"] in (256, 512, 1024)
     assert (cfg["panel_chunks"], cfg["rows_in_flight"]) in ((1, 2), (1, 4), (2, 2))
-    assert all(b in (64, 112, 128, 224, 256, 448, 512) for b in blocks)
+    # the powers of two and the multiples of the deep pass's window: 112
+    # columns at eight sweeps per round (1024-thread workgroups), 96 at
+    # sixteen (512 threads or fewer)
+    if cfg["wg_threads"] <= 512:
+        assert all(b in (64, 96, 128, 192, 256, 384, 512) for b in blocks)
+    else:
+        assert all(b in (64, 112, 128, 224, 256, 448, 512) for b in blocks)
     assert rate > 0
     tuned = [dict(p, block=b) for p, b in zip(probs, blocks)]
     out = cam.batch_run(tuned, wg_threads=cfg["wg_threads"],
