@@ -217,14 +217,21 @@ __device__ __noinline__ void do_round_deep(const double* in, double* out,
   if (RES) add_residual(res_slot + 2, acc);
 }
 
-// Rounds of tolerance problems: 3 .. kDeepTol sweeps of a tile in one pass
-// (kDeepTol - n_sw leading stages copy).  MODE 1: the exact residual of
-// every sweep j = 1 .. n_sw into res_slot[2(j-1) .. 2(j-1)+1] (the stop
-// rule needs each one); MODE 2: the FP64 sum of its squared changes into
-// resd_slot[j-1] instead (the stop filter); MODE 0: none (the repair of a
-// stop inside a round).
+// Rounds of tolerance problems: 3 .. D sweeps of a tile in one pass (D -
+// n_sw leading stages copy; D = kDeepTol, or kDeepTolWide in engines of up
+// to 512-thread workgroups, each engine with its own copy as for
+// do_round_deep).  MODE 1: the exact residual of every sweep j = 1 .. n_sw
+// into res_slot[2(j-1) .. 2(j-1)+1] (the stop rule needs each one); MODE 2:
+// the FP64 sum of its squared changes into resd_slot[j-1] instead (the stop
+// filter); MODE 0: none (the repair of a stop inside a round).
 constexpr int kDeepTol = 6;
-template <int MODE>
+constexpr int kDeepTolWide = 12;
+
+__host__ __device__ constexpr int deep_tol_for_threads(int threads) {
+  return threads <= 512 ? kDeepTolWide : kDeepTol;
+}
+
+template <int T, int D, int MODE>
 __device__ __noinline__ void do_round_tol(const double* in, double* out,
                                           int64_t pitch, int W, int H, int r0,
                                           int r1, int c0, int c1, int n_sw,
@@ -240,7 +247,6 @@ __device__ __noinline__ void do_round_tol(const double* in, double* out,
   c0 = uni(c0);
   c1 = uni(c1);
   n_sw = uni(n_sw);
-  constexpr int D = kDeepTol;
   constexpr int OW = jac::window_out_cols(D);
   const int rows = r1 - r0, S = (c1 - c0 + OW - 1) / OW;
   const int WV = blockDim.x / 64, wave = threadIdx.x / 64;
@@ -275,6 +281,24 @@ __device__ __noinline__ void do_round_tol(const double* in, double* out,
     for (int s = 0; s < D; s++)
       if (s >= n_copy) add_residual_d(resd_slot + (s - n_copy), accd[s]);
   }
+}
+
+// do_round_tol at the problem's round depth (kDeepTol, or kDeepTolWide in
+// engines of up to 512-thread workgroups).
+template <int T, int MODE>
+__device__ inline void round_tol(int depth, const double* in, double* out,
+                                 int64_t pitch, int W, int H, int r0, int r1,
+                                 int c0, int c1, int n_sw, uint64_t* res_slot,
+                                 double* resd_slot) {
+  if constexpr (deep_tol_for_threads(T) == kDeepTolWide) {
+    if (depth == kDeepTolWide) {
+      do_round_tol<T, kDeepTolWide, MODE>(in, out, pitch, W, H, r0, r1, c0,
+                                          c1, n_sw, res_slot, resd_slot);
+      return;
+    }
+  }
+  do_round_tol<T, kDeepTol, MODE>(in, out, pitch, W, H, r0, r1, c0, c1, n_sw,
+                                  res_slot, resd_slot);
 }
 
 // Hash of the tile's interior cells of array `src`.
@@ -640,10 +664,12 @@ __device__ void run_round(const Request& r, const TileGeom& g, int rr,
   uint64_t* slot = &p.res[tol ? (rr & 1) : 0][0][0];
   const int src = rr % 3, dst = (rr + 1) % 3;
   if (n_sw >= 3 && tol && filtered_round(r, rr)) {
-    do_round_tol<2>(arr(r, src), arr(r, dst), r.pitch, r.W, r.H, g.r0, g.r1,
+    round_tol<T, 2>(r.depth, arr(r, src), arr(r, dst), r.pitch, r.W, r.H,
+                    g.r0, g.r1,
                     g.c0, g.c1, n_sw, nullptr, &p.resd[rr & 1][0]);
   } else if (n_sw >= 3 && tol) {
-    do_round_tol<1>(arr(r, src), arr(r, dst), r.pitch, r.W, r.H, g.r0, g.r1,
+    round_tol<T, 1>(r.depth, arr(r, src), arr(r, dst), r.pitch, r.W, r.H,
+                    g.r0, g.r1,
                     g.c0, g.c1, n_sw, slot, nullptr);
   } else if (n_sw >= 3) {
     // every round but the last sweeps `depth` times; a shorter last round
@@ -754,7 +780,8 @@ __global__ void __launch_bounds__(T) engine_kernel(EngineDesc E) {
     } else if (kind == K_FIX && uni(s_n) < 0) {
       // round k again from its intact input, now with exact residuals (the
       // grid it writes is bit-identical to what the round wrote)
-      do_round_tol<1>(arr(r, k % 3), arr(r, (k + 1) % 3), r.pitch, r.W, r.H,
+      round_tol<T, 1>(r.depth, arr(r, k % 3), arr(r, (k + 1) % 3), r.pitch,
+                      r.W, r.H,
                       g.r0, g.r1, g.c0, g.c1,
                       min(r.depth, r.max_iters - r.depth * k),
                       &p.res[k & 1][0][0], nullptr);
@@ -766,8 +793,9 @@ __global__ void __launch_bounds__(T) engine_kernel(EngineDesc E) {
       else if (n == 2)
         do_round2<NC, U, false, false>(r, g, k % 3, (k + 1) % 3, nullptr);
       else
-        do_round_tol<0>(arr(r, k % 3), arr(r, (k + 1) % 3), r.pitch, r.W,
-                        r.H, g.r0, g.r1, g.c0, g.c1, n, nullptr, nullptr);
+        round_tol<T, 0>(r.depth, arr(r, k % 3), arr(r, (k + 1) % 3),
+                        r.pitch, r.W, r.H, g.r0, g.r1, g.c0, g.c1, n, nullptr,
+                        nullptr);
     } else {
       // the result of `k` sweeps is in array ceil(k / depth) % 3
       const uint64_t h = do_hash(r, g, ((k + r.depth - 1) / r.depth) % 3);
@@ -968,12 +996,13 @@ static Built build_request(const chm_batch_problem& pb, int block, int deep) {
   r.tol2 = pb.tol > 0 ? pb.tol * pb.tol : -1.0;
   r.seed = pb.seed;
   // rounds of `deep` (sixteen or eight) fused sweeps for a fixed sweep
-  // count, six with a tolerance (the pass then sums every sweep's residual,
-  // in registers); tiles and grid at least that many cells on a side, else
-  // eight, else two
+  // count, twelve or six with a tolerance (the pass then sums every sweep's
+  // residual, in registers); tiles and grid at least that many cells on a
+  // side, else eight (six), else two
   const int side = std::min(std::min(r.W, r.H), r.bs);
+  const int deep_tol = deep == kDeepWide ? kDeepTolWide : kDeepTol;
   if (r.tol2 >= 0)
-    r.depth = side >= kDeepTol ? kDeepTol : 2;
+    r.depth = side >= deep_tol ? deep_tol : (side >= kDeepTol ? kDeepTol : 2);
   else if (side >= deep)
     r.depth = deep;
   else
@@ -1504,9 +1533,10 @@ struct Server {
 // into about as many tiles as there are workgroups, so one round of it
 // occupies the whole GPU; with k requests resident each gets about 1/(k+1)
 // of them, and tiles grow (fewer fences per cell) as the load rises.
-static int serve_block(int W, int H, int n_wg, int resident) {
+static int serve_block(int W, int H, int n_wg, int resident, int ow) {
   const int64_t target = std::max<int64_t>(1, n_wg / (1 + resident));
-  static const int kSides[] = {448, 224, 112};
+  // multiples of the tolerance pass's window (ow output columns)
+  const int kSides[] = {4 * ow, 2 * ow, ow};
   for (int bs : kSides) {
     const int64_t tiles = (int64_t)((W + bs - 1) / bs) * ((H + bs - 1) / bs);
     if (tiles >= target) return bs;
@@ -1852,7 +1882,9 @@ int chm_jacobi2d_server_submit(void* handle, const chm_batch_problem* pb,
   const bool engine = s->mode == CHM_SERVE_ENGINE;
   int block = s->block;
   if (engine && pb->block <= 0 && block <= 0)
-    block = serve_block(pb->width, pb->height, s->e.d.n_wg, (int)s->live.size());
+    block = serve_block(pb->width, pb->height, s->e.d.n_wg,
+                        (int)s->live.size(),
+                        jac::window_out_cols(deep_tol_for_threads(s->e.threads)));
   const int deep = engine ? deep_for_threads(s->e.threads) : kDeep;
   CHM_TRY(check_problem(*pb, block, deep));
   Built b = build_request(*pb, block, deep);

@@ -76,8 +76,8 @@ struct Prob {
   uint32_t round_tiles[2];  // tiles that finished round r (slot r & 1)
   uint32_t fix_sweeps;
   uint32_t pad0;
-  uint64_t res[2][8][2];  // [r & 1][sweep j of round r][lo, hi] residual sums
-  double resd[2][8];      // [r & 1][sweep j]: FP64 sums of squared changes
+  uint64_t res[2][16][2];  // [r & 1][sweep j of round r][lo, hi] residual sums
+  double resd[2][16];      // [r & 1][sweep j]: FP64 sums of squared changes
                           // (the stop filter of rounds between the first
                           // and the last)
   uint32_t resid_round;   // round recomputed with exact residuals (state 5)

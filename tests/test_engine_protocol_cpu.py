@@ -57,11 +57,13 @@ def sweep_region(src, r0, r1, c0, c1, H, W):
 
 def depth_for(W, H, bs, tol2, deep=8):
     """Sweeps per round (build_request): `deep` (16 for engines of up to
-    512-thread workgroups, else 8) for a fixed sweep count, 6 with a
-    tolerance; 8, then 2, when the tile or grid is thinner than that."""
+    512-thread workgroups, else 8) for a fixed sweep count, 12 (engines of
+    up to 512 threads) or 6 with a tolerance; 8 (6), then 2, when the tile
+    or grid is thinner than that."""
     side = min(W, H, bs)
     if tol2 >= 0:
-        return 6 if side >= 6 else 2
+        deep_tol = 12 if deep == 16 else 6
+        return deep_tol if side >= deep_tol else (6 if side >= 6 else 2)
     if side >= deep:
         return deep
     return 8 if side >= 8 else 2
@@ -86,8 +88,8 @@ class Problem:
         self.fix_round = None
         self.fix_sweeps = 0
         self.round_tiles = [0, 0]
-        self.res = [[0] * 8, [0] * 8]
-        self.resd = [[0.0] * 8, [0.0] * 8]
+        self.res = [[0] * 16, [0] * 16]
+        self.resd = [[0.0] * 16, [0.0] * 16]
         self.resid_round = None
         self.resid_tiles = 0
         self.held_tile = None
@@ -310,7 +312,7 @@ class Engine:
                 p.inflight += p.n
                 return "resid"
             p.round_tiles[sk] = 0
-            p.resd[sk] = [0.0] * 8
+            p.resd[sk] = [0.0] * 16
             return "gate"
         return self.decide_exact(p, k, sk)
 
@@ -334,8 +336,8 @@ class Engine:
             p.s_last, p.stop_iters, p.state = s, stop, 1
             return None
         p.round_tiles[sk] = 0
-        p.res[sk] = [0] * 8
-        p.resd[sk] = [0.0] * 8
+        p.res[sk] = [0] * 16
+        p.resd[sk] = [0.0] * 16
         p.state = 0
         return "gate"
 
@@ -403,11 +405,14 @@ def test_protocol_with_sixteen_deep_rounds(seed):
     # or more), eight or two below that, under random interleavings
     cases = [(40, 30, 16, 9, 0.0), (33, 17, 16, 37, 0.0), (40, 40, 16, 29, 0.0),
              (64, 48, 16, 50, 0.0), (20, 20, 8, 17, 0.0), (12, 12, 4, 5, 0.0),
-             (40, 40, 20, 16, 0.0), (50, 40, 16, 400, 3e-2)]
+             (40, 40, 20, 16, 0.0), (50, 40, 16, 400, 3e-2),
+             (24, 24, 12, 400, 0.5), (26, 19, 13, 400, 2e-2),
+             (45, 21, 12, 400, 7e-3), (30, 22, 12, 400, 0.3),
+             (24, 24, 8, 400, 5e-2), (12, 12, 12, 160, 1e-13)]
     rng = random.Random(seed)
     probs = [Problem(W, H, bs, it, tol, 700 * seed + i, deep=16)
              for i, (W, H, bs, it, tol) in enumerate(cases)]
-    assert {p.depth for p in probs} == {16, 8, 6, 2}
+    assert {p.depth for p in probs} == {16, 12, 8, 6, 2}
     Engine(probs, rng).run(max_overlap=12)
     for i, ((W, H, bs, it, tol), p) in enumerate(zip(cases, probs)):
         s = 700 * seed + i
@@ -418,14 +423,15 @@ def test_protocol_with_sixteen_deep_rounds(seed):
         assert (p.stop_iters, p.hash, p.s_last) == (n, O.grid_hash(g), s_last)
 
 
-def test_protocol_with_every_filtered_round_recomputed(monkeypatch):
+@pytest.mark.parametrize("deep", [8, 16])
+def test_protocol_with_every_filtered_round_recomputed(monkeypatch, deep):
     # the exact recompute, the held release and the deferred gate on every
     # filtered round (not only where a stop may lie) still give the oracle's
-    # answers
+    # answers, in rounds of six and of twelve
     monkeypatch.setattr(sys.modules[__name__], "ALWAYS_RECOMPUTE", True)
     rng = random.Random(11)
     cases = [c for c in CASES if c[4] > 0]
-    probs = [Problem(W, H, bs, it, tol, 500 + i)
+    probs = [Problem(W, H, bs, it, tol, 500 + i, deep=deep)
              for i, (W, H, bs, it, tol) in enumerate(cases)]
     Engine(probs, rng).run(max_overlap=12)
     for i, ((W, H, bs, it, tol), p) in enumerate(zip(cases, probs)):

@@ -67,6 +67,38 @@ def test_stream_matches_oracle(cam, mode):
         assert c["latency_ms"] >= 0 and c["device_ms"] >= 0
 
 
+@pytest.mark.parametrize("threads", [512, 256])
+def test_twelve_deep_tolerance_rounds_match_oracle(cam, threads):
+    # engines of up to 512-thread workgroups run tolerance requests in rounds
+    # of twelve fused sweeps (six, then two, on thinner grids and tiles): stops
+    # inside a round are repaired, filtered rounds recomputed where a stop
+    # may lie; fixed-count requests go in rounds of sixteen
+    reqs = requests(20, 7 + threads)
+    for i, p in enumerate(reqs):
+        p["block"] = [0, 16, 24, 96, 192][i % 5]
+        p["iters"] = 20 + 37 * i
+    reqs.append({"width": 300, "height": 280, "iters": 77, "seed": 901})
+    print("server", threads, flush=True)
+    with cam.Server("engine", budget_bytes=1 << 30, wg_threads=threads) as srv:
+        tickets = {}
+        for p in reqs:
+            t = srv.submit(p)
+            assert t is not None
+            tickets[t] = p
+        got = drain(srv, tickets)
+    for t, p in tickets.items():
+        c = got[t]
+        if "tol" in p:
+            h, n, s_last, s0 = oracle_tol(p)
+            assert (c["hash"], c["iters"], c["residual"], c["residual0"]) == \
+                (h, n, s_last, s0), p
+        else:
+            g, res = O.jacobi_run(p["width"], p["height"], p["iters"],
+                                  seed=p["seed"])
+            assert (c["hash"], c["iters"], c["residual"]) == \
+                (O.grid_hash(g), p["iters"], res), p
+
+
 def test_result_alone_equals_in_mix(cam):
     reqs = requests(16, 2)
     with cam.Server("engine", budget_bytes=1 << 30) as srv:
