@@ -757,16 +757,23 @@ __device__ inline void sweep_panelD(const double* __restrict__ in,
   // q1-1+RIN.
   const int n_steps = r1 - r0 + 2 * D - 2;
   // Steps [tA, tB) form the steady state: every sweep is under way, no row
-  // of the grid's ring is met (the ring rows 0 and h+1 only occur among the
-  // first 2D-2 and the last D-1 steps) and no load runs past the band, so
-  // those steps run without guards or ring checks; a wave that holds a
-  // ring column runs them with that column's selects (RINGC, its own
-  // instantiation: a select test inside one shared steady loop raised the
-  // register pressure of the pass enough to spill in every step, 571 ms
-  // instead of 124 per 16384^2 solve) or else guarded.  Both bounds are
-  // multiples of U, as the ring slots require.
+  // of the grid's ring is met (row 0 only occurs among the first 2D-2
+  // steps, row h+1 only among the last D-1 and only if the band ends within
+  // D-2 rows of it: other bands run their drain in the steady state too,
+  // 109.5 -> 105.7 ms of sweeping per chare of a 16384^2 solve) and no load
+  // runs past the band, so those steps run without guards or ring checks;
+  // a wave that holds a ring column runs them with that column's selects
+  // (RINGC, its own instantiation: a select test inside one shared steady
+  // loop raised the register pressure of the pass enough to spill in every
+  // step, 571 ms instead of 124 per 16384^2 solve) or else guarded.  Both
+  // bounds are multiples of U, as the ring slots require.  (Running the
+  // pipeline fill in the steady loop as well, the sweeps not yet under way
+  // computing rows nothing reads, measured slower: 107.9 against 105.7 ms;
+  // it does every stage's work in every fill step.)
   const int tA = (2 * D - 2 + U - 1) / U * U;
-  const int t_hi = min(n_steps - (D - 1), n_steps + 2 - RIN);
+  const bool ring_below = rs.bottom && h + 1 <= r1 + D - 2;
+  const int t_hi = ring_below ? min(n_steps - (D - 1), n_steps + 2 - RIN)
+                              : n_steps + 2 - RIN;
   int tB = tA + (t_hi - tA) / U * U;
   if ((!RINGC && p.ringcols) || tB < tA) tB = tA;
   auto guarded = [&](int t0) {

@@ -121,10 +121,13 @@ def sweep_window(D, inp, out, w, h, c0, r0, r1, rs, acc, eo, pack, rin=3,
     for k in range(rin):
         if r0 - D + k <= last_in:
             load(R[k], r0 - D + k)
-    S = [[np.zeros((64, 2)) for _ in range(3)] for _ in range(D - 1)]
+    # NaN: a row read before it is made would reach the result
+    S = [[np.full((64, 2), np.nan) for _ in range(3)] for _ in range(D - 1)]
     n_steps = r1 - r0 + 2 * D - 2
     tA = (2 * D - 2 + U - 1) // U * U
-    t_hi = min(n_steps - (D - 1), n_steps + 2 - rin)
+    ring_below = bottom and h + 1 <= r1 + D - 2
+    t_hi = (min(n_steps - (D - 1), n_steps + 2 - rin) if ring_below
+            else n_steps + 2 - rin)
     x = t_hi - tA
     tB = tA + int(x / U) * U   # C's division truncates toward zero
     if (not ringc and ringcols) or tB < tA:
@@ -155,7 +158,8 @@ def sweep_window(D, inp, out, w, h, c0, r0, r1, rs, acc, eo, pack, rin=3,
         while t0 < tB:
             for k in range(U):
                 ks = k % 3
-                q1 = r0 - D + 1 + t0 + k
+                t = t0 + k
+                q1 = r0 - D + 1 + t
                 # the steady state must never meet what it skips
                 assert not ((q1 == 0 and top) or (q1 == h + 1 and bottom))
                 S[0][ks] = stage(R[k % rin], R[(k + 1) % rin], R[(k + 2) % rin],
