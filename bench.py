@@ -333,8 +333,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--tune-iters", type=int, default=32,
-                    help="sweeps per problem in the autotuner's samples")
+    ap.add_argument("--tune-iters", type=int, default=0,
+                    help="sweeps per problem in the autotuner's samples "
+                         "(0 = every problem's own count)")
     ap.add_argument("--serve-plan", default="10:100,50:250,100:1000,500:300,"
                     "2000:1000", help="rate:requests,... of the serving block")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)

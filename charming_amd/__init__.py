@@ -302,7 +302,7 @@ def batch_plain(probs, use_graph=False):
     return _pack(res, h, r, it)
 
 
-def batch_autotune(probs, sample_iters=32):
+def batch_autotune(probs, sample_iters=0):
     """Returns (config dict, per-problem tile sides, cell updates/s)."""
     arr = _problems(probs)
     cfg = BatchConfig()

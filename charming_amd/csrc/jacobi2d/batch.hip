@@ -1726,8 +1726,12 @@ int chm_jacobi2d_batch_autotune(const chm_batch_problem* probs, int n,
                                 int* blocks_out, double* best_rate) {
   if (!probs || !best || n < 1) return fail(CHM_E_INVALID, "null argument");
   // whole rounds for every candidate: a sample that ends inside a round of
-  // sixteen pays for the round's copying stages and misranks the engines
-  const int cap = ((sample_iters > 0 ? sample_iters : 32) + 15) / 16 * 16;
+  // sixteen pays for the round's copying stages and misranks the engines.
+  // 0: no cap.  (Each timed run's setup, seeding and readback cost more than
+  // its sweeps on the 512-problem batch, so full-length runs tune in less
+  // time than 32-sweep samples, 16.4 against 21.8 s, and rank better: their
+  // pick ran 102.6-103.2 ms, the 32-sweep samples' pick 114-122 ms.)
+  const int cap = sample_iters > 0 ? (sample_iters + 15) / 16 * 16 : 0;
   // tile sides tried per size class: the powers of two and the multiples
   // of the deep pass's window (112 columns at eight sweeps, 96 at sixteen)
   static const int kBlocks8[] = {64, 112, 128, 224, 256, 448, 512};

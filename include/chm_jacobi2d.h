@@ -228,8 +228,8 @@ int chm_jacobi2d_batch_plain(const chm_batch_problem* probs, int n,
  * workgroups of 512 threads or fewer, {64, 112, 128, 224, 256, 448, 512}
  * for 1024, by coordinate descent over size classes, timing short engine
  * runs of the whole batch with every problem capped at sample_iters sweeps
- * rounded up to a multiple of 16 (0 = 32).  best_rate = cell updates per
- * second of the best. */
+ * rounded up to a multiple of 16 (0 = no cap: full-length runs).  best_rate
+ * = cell updates per second of the best. */
 int chm_jacobi2d_batch_autotune(const chm_batch_problem* probs, int n,
                                 int sample_iters, chm_batch_config* best,
                                 int* blocks_out, double* best_rate);
