@@ -243,7 +243,8 @@ typedef struct chm_server_config {
   int mode;            /* enum chm_server_mode */
   size_t budget_bytes; /* HBM for resident grids (0 = 16 GiB) */
   int max_active;      /* requests resident at once (0 = 4096) */
-  int wg_threads, block, panel_chunks, rows_in_flight; /* as chm_batch_config;
+  int wg_threads, block, panel_chunks, rows_in_flight; /* as chm_batch_config
+                          (wg_threads 0 = 512: tolerance rounds of twelve);
                           block 0 = by load: a request gets about
                           workgroups / (1 + resident requests) tiles */
 } chm_server_config;
