@@ -268,7 +268,10 @@ __device__ __noinline__ void do_round_tol(const double* in, double* out,
   while (pos < end) {
     const int panel = (int)(pos / rows), rr = (int)(pos % rows);
     const int len = (int)min((int64_t)(rows - rr), end - pos);
-    jac::sweep_panelD<D, MODE == 1, false, MODE>(
+    // (the twelve-deep filtered rounds, most of a request's, keep four rows
+    // of loads in flight: 141 against 125 solves/s at saturation)
+    jac::sweep_panelD<D, MODE == 1, false, MODE,
+                      (D == kDeepTolWide && MODE == 2) ? 6 : 3>(
         in, out, pitch, W, H, c0 + panel * OW, r0 + rr, r0 + rr + len,
         accs[D - 1], none, rs, c1 - 1, n_copy, accs, accd);
     pos += len;
@@ -1849,7 +1852,7 @@ int chm_jacobi2d_server_start(const chm_server_config* cfg, void** handle) {
   if (s->mode == CHM_SERVE_ENGINE) {
     // (default 512 threads: tolerance rounds of twelve sweeps; on a
     // saturating stream 124.8 solves/s against 119.4 at 1024 threads and
-    // rounds of six)
+    // rounds of six, before the filtered rounds' six-slot input ring)
     const int threads = cfg->wg_threads ? cfg->wg_threads : 512;
     const int nc = cfg->panel_chunks ? cfg->panel_chunks : 1;
     const int u = cfg->rows_in_flight ? cfg->rows_in_flight : 2;
